@@ -1,0 +1,132 @@
+/* C ABI of libinductioneq_hip.so -- the MI355X (gfx950) implementation of the
+ * InductionEq right-hand side, Runge-Kutta stages and CG divergence cleaning.
+ *
+ * The reference drives one OpenCL program per run through MatCL (an external MATLAB
+ * mex bridge): a build call with a source-module list and a -D settings string, then
+ * run calls that execute a list of kernel names, in order, on one argument list.
+ * This header keeps that contract: ieq_program_build takes the same settings string
+ * and module list, ieq_run takes the same kernel names with the same argument order.
+ * Arguments are device pointers; every entry point returns IEQ_OK (0) or a negative
+ * error code, with the message available from ieq_last_error.
+ *
+ * Device field layout: a vector field (the reference's REAL4 array) is stored as
+ * three planes of `stride` REALs (x, y, z components; the reference's unused fourth
+ * lane is not stored); a scalar field (REAL array) is one plane; the node index is
+ * ix + NX*(iy + NY*iz) as in the reference (include/utils.h:66-68).  `stride` is the
+ * node count rounded up to 64 and is reported by ieq_program_get_info.  `time` is a
+ * device REAL[2] with time[0] the time at the start of a step and time[1] the time
+ * of the current stage (kernel_time_integrator.cl:51-55).
+ */
+#ifndef INDUCTIONEQ_H
+#define INDUCTIONEQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ieq_program ieq_program;
+
+enum {
+  IEQ_OK = 0,
+  IEQ_ERR_SETTINGS = -1,    /* invalid settings or module list (the reference's #error / build failure) */
+  IEQ_ERR_KERNEL = -2,      /* kernel name not in the program (CL_INVALID_KERNEL_NAME) */
+  IEQ_ERR_ARGS = -3,        /* wrong argument count, null pointer or NDRange outside the grid */
+  IEQ_ERR_HIP = -4,         /* HIP runtime error */
+  IEQ_ERR_UNSUPPORTED = -5  /* option combination this build does not provide */
+};
+
+/* Parsed program settings (read-only view). */
+typedef struct {
+  int real_bytes;     /* 8 = double, 4 = float (REAL) */
+  int opset;          /* 0 2ndOrder, 1 4thOrder, 2 6thOrder, 3 2ndOrderExtended, 4 4thOrderExtended, 5 6thOrderExtended */
+  int order;          /* ORDER */
+  int num_bounds;     /* NUM_BOUNDS */
+  int stencil_width;  /* STENCIL_WIDTH */
+  int nx, ny, nz;     /* NODES_X/Y/Z */
+  double dx, dy, dz, xmin, ymin, zmin, dt;
+  int periodic;       /* USE_PERIODIC */
+  int hall;           /* USE_HALL */
+  int dissipation;    /* 0 none, 1 high order, 2 first order, 3 adaptive */
+  int form_uibj, form_source, form_ujbi;
+  int testcase;       /* 0 rotation_2D, 1 rotation_3D, 2 hall_periodic, 3 hall_travelling_wave, 4 confined_domain,
+                         5 alfven_periodic_2D, 6 gaussian_rotation_3D, 7 uniform_flow */
+  int laplace;        /* 0 WS_D0, 1 NS_D0, 2 WS_LN, -1 no divergence-cleaning module */
+  int bnodes;         /* BNODES */
+  int w_size;         /* W_SIZE */
+  double ho_factor, mp2min, mp2max, cmin, cmax, ux, uy, uz;
+  long long num_nodes, num_nodes_pad, stride, num_groups;
+} ieq_program_info;
+
+/* Build a program.  Replaces MatCL's build call cl_run_kernel(device, kernel_path_list,
+ * settings) (matlab/+induction_eq/compute_numerical_solution.m:23; module list and
+ * settings assembled at compute_numerical_solution_setup.m:57-137, initialize.m:63-77,
+ * divergence_cleaning/initialize.m:105-154).  `settings` is the generate_settings
+ * string (matlab/generate_settings.m:3-30); `modules` are the source names of
+ * kernel_path_list with or without directory and extension ("4thOrder",
+ * "../include/rotation_2D.h", "kernel_time_integrator", ...).  Does not touch the GPU. */
+int ieq_program_build(ieq_program **prog, const char *settings, const char *const *modules, int nmod);
+
+/* Release a program and the HIP objects it created. */
+void ieq_program_release(ieq_program *prog);
+
+/* Message of the last error on `prog` (or of the last failed build when prog is NULL). */
+const char *ieq_last_error(const ieq_program *prog);
+
+/* Parsed settings of a program. */
+int ieq_program_get_info(const ieq_program *prog, ieq_program_info *info);
+
+/* Stream on which all work of `prog` is enqueued (a hipStream_t; NULL = default stream). */
+int ieq_set_stream(ieq_program *prog, void *hip_stream);
+
+/* Wait for all work enqueued on the program's stream. */
+int ieq_synchronize(ieq_program *prog);
+
+/* Device allocation helpers (zero-filled).  Sizes: ieq_vector_field_bytes /
+ * ieq_scalar_field_bytes for one field of this program. */
+int ieq_alloc(ieq_program *prog, size_t bytes, void **dptr);
+int ieq_free(ieq_program *prog, void *dptr);
+size_t ieq_vector_field_bytes(const ieq_program *prog);
+size_t ieq_scalar_field_bytes(const ieq_program *prog);
+
+/* Host <-> device copies in the reference host layout: `n` nodes of AoS REAL4
+ * (x, y, z, w) <-> a device vector field; `n` REALs <-> a device scalar field.
+ * Replace MatCL's implicit copy-in / write-back of MATLAB arrays
+ * (matlab/+induction_eq/initialize.m:88-93). Downloads write w = 0. */
+int ieq_upload_real4(ieq_program *prog, void *dst, const void *host_aos, size_t n);
+int ieq_download_real4(ieq_program *prog, void *host_aos, const void *src, size_t n);
+int ieq_upload_real(ieq_program *prog, void *dst, const void *host, size_t n);
+int ieq_download_real(ieq_program *prog, void *host, const void *src, size_t n);
+
+/* Run a list of reference kernels, in order, on one argument list.  Replaces MatCL's
+ * run call t = cl_run_kernel(device, names, g_range, l_range, args..., 0)
+ * (compute_numerical_solution.m:39-41,95-97; clean_div_CG.m:31-79).  Kernel names and
+ * argument orders are those of kernel/kernel_time_integrator.cl (95 stage, time,
+ * velocity and curl kernels), kernel_init.cl, kernel_operator.cl,
+ * kernel_div_cleaning.cl, kernel_CG.cl and kernel_dot_product.cl.  `g` is the NDRange,
+ * `l` the local size (only the reductions use it: one output per l[0] nodes).
+ * Scalars such as the CG coefficient are 1-element device buffers (kernel_CG.cl:6-7).
+ * When kernel_seconds is non-NULL the call waits for the kernels and stores their time. */
+int ieq_run(ieq_program *prog, const char *const *names, int n, const uint32_t g[3], const uint32_t l[3], void *const *args,
+            int nargs, double *kernel_seconds);
+
+/* Advance `nsteps` full time steps of `integrator` (SSPRK33, SSPRK104,
+ * KennedyCarpenterLewis2R54C, CalvoFrancoRandez2R64, CarpenterKennedy2N54,
+ * ToulorgeDesmet2N84F) with the integrator's reference argument list
+ * (b1, b2[, b3], curlB_rho, u, rho, time).  Equivalent to ieq_run on the repeated
+ * RK_Step list of compute_numerical_solution_setup.m:140-371 (block loop of
+ * compute_numerical_solution.m:91-108); when the grid is periodic and u and rho are
+ * uniform it runs the fused stage kernels instead of the reference kernel sequence.
+ * On return b1 and time hold the same state as after the reference schedule; b2/b3
+ * and curlB_rho are work registers whose contents are unspecified.
+ * `variable_u` requests calc_u before each stage (I_RunOps('variable_u')). */
+int ieq_step(ieq_program *prog, const char *integrator, long long nsteps, int variable_u, void *const *args, int nargs,
+             double *kernel_seconds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INDUCTIONEQ_H */

@@ -1,0 +1,95 @@
+"""ctypes binding of ``libinductioneq_hip.so`` (declared in ``include/inductioneq.h``).
+
+There is no fallback: if the library has not been built, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libinductioneq_hip.so")
+
+IEQ_OK = 0
+IEQ_ERR_SETTINGS = -1
+IEQ_ERR_KERNEL = -2
+IEQ_ERR_ARGS = -3
+IEQ_ERR_HIP = -4
+IEQ_ERR_UNSUPPORTED = -5
+
+# every entry point declared in include/inductioneq.h
+EXPORTS = [
+    "ieq_program_build", "ieq_program_release", "ieq_last_error", "ieq_program_get_info", "ieq_set_stream",
+    "ieq_synchronize", "ieq_alloc", "ieq_free", "ieq_vector_field_bytes", "ieq_scalar_field_bytes",
+    "ieq_upload_real4", "ieq_download_real4", "ieq_upload_real", "ieq_download_real", "ieq_run", "ieq_step",
+]
+
+
+class ProgramInfo(ctypes.Structure):
+    _fields_ = [
+        ("real_bytes", ctypes.c_int), ("opset", ctypes.c_int), ("order", ctypes.c_int), ("num_bounds", ctypes.c_int),
+        ("stencil_width", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("xmin", ctypes.c_double),
+        ("ymin", ctypes.c_double), ("zmin", ctypes.c_double), ("dt", ctypes.c_double),
+        ("periodic", ctypes.c_int), ("hall", ctypes.c_int), ("dissipation", ctypes.c_int),
+        ("form_uibj", ctypes.c_int), ("form_source", ctypes.c_int), ("form_ujbi", ctypes.c_int),
+        ("testcase", ctypes.c_int), ("laplace", ctypes.c_int), ("bnodes", ctypes.c_int), ("w_size", ctypes.c_int),
+        ("ho_factor", ctypes.c_double), ("mp2min", ctypes.c_double), ("mp2max", ctypes.c_double),
+        ("cmin", ctypes.c_double), ("cmax", ctypes.c_double), ("ux", ctypes.c_double), ("uy", ctypes.c_double),
+        ("uz", ctypes.c_double), ("num_nodes", ctypes.c_longlong), ("num_nodes_pad", ctypes.c_longlong),
+        ("stride", ctypes.c_longlong), ("num_groups", ctypes.c_longlong),
+    ]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library; raises if it is missing (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libinductioneq_hip.so is not built (%s); run `python -m inductioneq_amd.build` "
+                          "or __graft_entry__.build()" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    lib.ieq_program_build.argtypes = [ctypes.POINTER(P), ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int]
+    lib.ieq_program_build.restype = ctypes.c_int
+    lib.ieq_program_release.argtypes = [P]
+    lib.ieq_program_release.restype = None
+    lib.ieq_last_error.argtypes = [P]
+    lib.ieq_last_error.restype = ctypes.c_char_p
+    lib.ieq_program_get_info.argtypes = [P, ctypes.POINTER(ProgramInfo)]
+    lib.ieq_set_stream.argtypes = [P, P]
+    lib.ieq_synchronize.argtypes = [P]
+    lib.ieq_alloc.argtypes = [P, ctypes.c_size_t, ctypes.POINTER(P)]
+    lib.ieq_free.argtypes = [P, P]
+    lib.ieq_vector_field_bytes.argtypes = [P]
+    lib.ieq_vector_field_bytes.restype = ctypes.c_size_t
+    lib.ieq_scalar_field_bytes.argtypes = [P]
+    lib.ieq_scalar_field_bytes.restype = ctypes.c_size_t
+    for fn in ("ieq_upload_real4", "ieq_download_real4", "ieq_upload_real", "ieq_download_real"):
+        getattr(lib, fn).argtypes = [P, P, P, ctypes.c_size_t]
+    u32x3 = ctypes.POINTER(ctypes.c_uint32)
+    lib.ieq_run.argtypes = [P, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, u32x3, u32x3, ctypes.POINTER(P),
+                            ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    lib.ieq_step.argtypes = [P, ctypes.c_char_p, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(P), ctypes.c_int,
+                             ctypes.POINTER(ctypes.c_double)]
+    for fn in EXPORTS:
+        if fn not in ("ieq_program_release", "ieq_last_error", "ieq_vector_field_bytes", "ieq_scalar_field_bytes"):
+            getattr(lib, fn).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+class IEQError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+def check(rc: int, handle=None):
+    if rc != IEQ_OK:
+        msg = load().ieq_last_error(handle)
+        raise IEQError(rc, msg.decode() if msg else "error")

@@ -1,0 +1,28 @@
+// Fused time stepping for periodic grids with uniform velocity and density.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/inductioneq.h"
+
+namespace ieq {
+
+struct Params;
+struct FusedWork;
+
+// Kernel names of one time step of `integrator` (compute_numerical_solution_setup.m:140-371).
+// Returns 0, or -1 for an unknown integrator.
+int schedule(const char* integrator, int hall, int variable_u, std::vector<std::string>& names, int& nf);
+
+// Advance nsteps with the fused kernels when the program qualifies (periodic grid,
+// uniform u and rho, no time-dependent velocity, no dissipation other than high
+// order).  Sets *used = 0 and enqueues nothing when it does not qualify.
+int fused_step(const Params& p, const ieq_program_info& info, hipStream_t s, FusedWork** work, const char* integrator,
+               long long nsteps, int variable_u, void* const* args, int* used, std::string& err);
+
+void fused_release(FusedWork* w);
+
+}  // namespace ieq

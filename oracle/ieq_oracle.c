@@ -1,0 +1,891 @@
+/* CPU ORACLE -- TEST INFRASTRUCTURE, NOT THE PRODUCT.
+ *
+ * A kernel-by-kernel C restatement of the InductionEq OpenCL program, kept in the
+ * reference's own structure so that it can serve both as the parity checker for
+ * the HIP library and as the reference-schedule CPU baseline:
+ *   - fields are AoS REAL4 (x, y, z, w) per node, linear index ix + NX*(iy + NY*iz)
+ *     (include/utils.h:66-68), zero padded to num_nodes_pad;
+ *   - every kernel loops over its NDRange one node at a time, selects operator rows
+ *     per axis exactly as get_bound_* (include/utils.h:125-136,172-201) and fetches
+ *     neighbours with the periodic wrap / centre-clamp rule of get_vector_field
+ *     (include/utils.h:89-103,142-153);
+ *   - kernels are dispatched by their reference names with the reference argument
+ *     order (kernel/kernel_time_integrator.cl, kernel_init.cl, kernel_operator.cl,
+ *     kernel_div_cleaning.cl, kernel_CG.cl, kernel_dot_product.cl).
+ * Compiled twice: -DREAL=double (ieq_oracle_f64.so) and -DREAL=float (ieq_oracle_f32.so).
+ * Only tests/, the smoke check and the benchmark's cpu_baseline leg load it.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../inductioneq_amd/csrc/ieq_opsets.h"
+
+#ifndef REAL
+#define REAL double
+#endif
+
+typedef struct { REAL x, y, z, w; } real4;
+
+/* ---- configuration: the reference's compile-time -D settings, as run-time data ---- */
+typedef struct {
+  int opset;                 /* IEQ_OPSET_* (order header) */
+  int nx, ny, nz;            /* NODES_X/Y/Z */
+  double dx, dy, dz;         /* DX, DY, DZ */
+  double xmin, ymin, zmin;   /* XMIN, YMIN, ZMIN */
+  double dt;                 /* DT */
+  int periodic;              /* USE_PERIODIC */
+  int hall;                  /* USE_HALL */
+  int dissipation;           /* 0 none, 1 USE_HIGH_ORDER_, 2 USE_FIRST_ORDER_, 3 USE_ADAPTIVE_DISSIPATION */
+  double ho_factor;          /* HO_DISSIPATION_FACTOR (default 1) */
+  double mp2min, mp2max, cmin, cmax;
+  int form_uibj;             /* 0 CENTRAL, 1 SPLIT, 2 PRODUCT */
+  int form_source;           /* 0 ZERO, 1 CENTRAL, 2 SPLIT */
+  int form_ujbi;             /* 0 CENTRAL, 1 SPLIT, 2 PRODUCT */
+  int testcase;              /* see IEQO_TC_* */
+  int laplace;               /* IEQ_LAP_WS_D0 / NS_D0 / WS_LN */
+  int bnodes;                /* BNODES */
+  double ux, uy, uz;         /* uniform velocity of the "uniform_flow" testcase */
+  int nthreads;              /* OpenMP threads, 0 = runtime default */
+} ieqo_cfg;
+
+enum {
+  IEQO_TC_ROTATION_2D = 0,
+  IEQO_TC_ROTATION_3D = 1,
+  IEQO_TC_HALL_PERIODIC = 2,
+  IEQO_TC_HALL_TRAVELLING_WAVE = 3,
+  IEQO_TC_CONFINED_DOMAIN = 4,
+  IEQO_TC_ALFVEN_PERIODIC_2D = 5,
+  IEQO_TC_GAUSSIAN_ROTATION_3D = 6,
+  IEQO_TC_UNIFORM_FLOW = 7
+};
+
+typedef struct {
+  const ieqo_cfg *c;
+  const ieq_opset_desc *op;
+  uint32_t nx, ny, nz;
+} ctx_t;
+
+/* ---- REAL4 arithmetic ---- */
+static inline real4 r4(REAL x, REAL y, REAL z, REAL w) { real4 r = {x, y, z, w}; return r; }
+static inline real4 add4(real4 a, real4 b) { return r4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+static inline real4 sub4(real4 a, real4 b) { return r4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
+static inline real4 mul4(real4 a, real4 b) { return r4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w); }
+static inline real4 smul4(REAL s, real4 a) { return r4(s * a.x, s * a.y, s * a.z, s * a.w); }
+static inline real4 sdiv4(real4 a, REAL s) { return r4(a.x / s, a.y / s, a.z / s, a.w / s); }
+static inline real4 zero4(void) { return r4(0, 0, 0, 0); }
+static inline real4 fabs4(real4 a) { return r4(fabs(a.x), fabs(a.y), fabs(a.z), fabs(a.w)); }
+static inline real4 fmax4(real4 a, real4 b) { return r4(fmax(a.x, b.x), fmax(a.y, b.y), fmax(a.z, b.z), fmax(a.w, b.w)); }
+static inline REAL dot4(real4 a, real4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+
+/* ---- index helpers: include/utils.h ---- */
+static inline uint32_t calc_idx(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz) {
+  return ix + iy * k->nx + iz * k->ny * k->nx; /* utils.h:66-68 */
+}
+
+/* Row of a (2nb+1)-row operator table for node i on an axis of n nodes (utils.h:125-136,172-201). */
+static inline int bound_row(uint32_t i, uint32_t n, int nb, int periodic) {
+  int b = 0;
+  if (periodic) return nb;
+  for (int k = 0; k < nb; k++) b += (nb - k) * ((int)(i == n - (uint32_t)(k + 1)) - (int)(i == (uint32_t)k));
+  return nb + b;
+}
+
+/* Neighbour index along one axis (utils.h:91-98 periodic wrap, 144-148 centre clamp). */
+static inline uint32_t nbr(uint32_t i, int off, uint32_t n, int periodic) {
+  uint32_t a = (uint32_t)(off < 0 ? -off : off);
+  if (periodic) {
+    uint32_t j = i + (uint32_t)off;
+    if (off < 0 && !(i > a - 1)) j += n;
+    if (off > 0 && !(i < n - a)) j -= n;
+    return j;
+  }
+  if (off < 0) return (i > a - 1) ? i + (uint32_t)off : i;
+  if (off > 0) return (i < n - a) ? i + (uint32_t)off : i;
+  return i;
+}
+
+static inline uint32_t at(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, int bx, int by, int bz) {
+  int p = k->c->periodic;
+  return calc_idx(k, nbr(ix, bx, k->nx, p), nbr(iy, by, k->ny, p), nbr(iz, bz, k->nz, p));
+}
+
+#define OPD(k, row, i) ((REAL)(k)->op->diff[(row) * (k)->op->w + (i)])
+#define OPDADJ(k, row, i) ((REAL)(k)->op->diff_adj[(row) * (k)->op->w + (i)])
+
+/* ---- testcases: include/<testcase>.h ---- */
+static void node_xyz(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, REAL *x, REAL *y, REAL *z) {
+  *x = (REAL)k->c->xmin + ix * (REAL)k->c->dx;
+  *y = (REAL)k->c->ymin + iy * (REAL)k->c->dy;
+  *z = (REAL)k->c->zmin + iz * (REAL)k->c->dz;
+}
+
+static real4 tc_b_analytical(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, REAL t) {
+  REAL x, y, z;
+  node_xyz(k, ix, iy, iz, &x, &y, &z);
+  switch (k->c->testcase) {
+    case IEQO_TC_ROTATION_2D: { /* rotation_2D.h:15-32 */
+      REAL s = sin(t), c = cos(t);
+      REAL xx = c * x + s * y, yy = -s * x + c * y;
+      REAL fac = exp(-20 * ((xx - (REAL)0.5) * (xx - (REAL)0.5) + yy * yy));
+      REAL bx0 = -4 * yy * fac, by0 = (4 * xx - 2) * fac;
+      return r4(c * bx0 - s * by0, s * bx0 + c * by0, 0, 0);
+    }
+    case IEQO_TC_GAUSSIAN_ROTATION_3D: {
+      /* B0 = curl (0, 0, exp(-20((x-1/2)^2 + y^2 + z^2))), rigidly rotated about z. */
+      REAL s = sin(t), c = cos(t);
+      REAL xx = c * x + s * y, yy = -s * x + c * y;
+      REAL fac = exp(-20 * ((xx - (REAL)0.5) * (xx - (REAL)0.5) + yy * yy + z * z));
+      REAL bx0 = -40 * yy * fac, by0 = (40 * xx - 20) * fac;
+      return r4(c * bx0 - s * by0, s * bx0 + c * by0, 0, 0);
+    }
+    case IEQO_TC_ROTATION_3D: { /* rotation_3D.h:7-33 */
+      REAL s = sin(t), c = cos(t), sq3 = sqrt(3.0);
+      REAL xx = ((1 + 2 * c) * x + (1 - c + sq3 * s) * y + (1 - c - sq3 * s) * z) / 3.0;
+      REAL yy = ((1 - c - sq3 * s) * x + (1 + 2 * c) * y + (1 - c + sq3 * s) * z) / 3.0;
+      REAL zz = ((1 - c + sq3 * s) * x + (1 - c - sq3 * s) * y + (1 + 2 * c) * z) / 3.0;
+      REAL alpha = exp(-(REAL)(5.0 / 3.0) * (3 - 2 * (3 + sq3) * xx + 12 * xx * xx - 2 * (-3 + sq3) * yy + 12 * yy * yy +
+                                              4 * sq3 * zz + 12 * zz * zz));
+      REAL b0x = alpha * (3 - sq3 - 4 * sq3 * yy + 4 * sq3 * zz) / 48.0;
+      REAL b0y = alpha * (-3 - sq3 + 4 * sq3 * xx - 4 * sq3 * zz) / 48.0;
+      REAL b0z = alpha * (1 - 2 * xx + 2 * yy) / (8 * sq3);
+      REAL bx = ((1 + 2 * c) * b0x + (1 - c - sq3 * s) * b0y + (1 - c + sq3 * s) * b0z) / 3.0;
+      REAL by = ((1 - c + sq3 * s) * b0x + (1 + 2 * c) * b0y + (1 - c - sq3 * s) * b0z) / 3.0;
+      REAL bz = ((1 - c - sq3 * s) * b0x + (1 - c + sq3 * s) * b0y + (1 + 2 * c) * b0z) / 3.0;
+      return r4(bx, by, bz, 0);
+    }
+    case IEQO_TC_HALL_PERIODIC: { /* hall_periodic.h:3-34 */
+      const REAL n1 = (REAL)0.5773502691896258, n2 = n1, n3 = n1, a = 1, b = 1, cc = 1, al = (REAL)0.5;
+      REAL kk = (REAL)(1.0 - al * al) / al;
+      REAL u1 = a * cos(kk * y + al * kk * t * n2) + b * sin(kk * z + al * kk * t * n3);
+      REAL u2 = b * cos(kk * z + al * kk * t * n3) + cc * sin(kk * x + al * kk * t * n1);
+      REAL u3 = cc * cos(kk * x + al * kk * t * n1) + a * sin(kk * y + al * kk * t * n2);
+      return r4(al * u1 + n1, al * u2 + n2, al * u3 + n3, 0);
+    }
+    case IEQO_TC_HALL_TRAVELLING_WAVE: { /* hall_travelling_wave.h:3-30 */
+      const REAL B0 = 1, Bz = 1, uz = (REAL)0.1, kk = 1, om = 1;
+      REAL xi = kk * z - om * t;
+      REAL Om = ((kk * uz - om) * (kk * uz - om) - kk * kk * Bz * Bz) / (Bz * (kk * uz - om) * kk * kk);
+      return r4(B0 * cos(Om * xi), B0 * sin(Om * xi), Bz, 0);
+    }
+    case IEQO_TC_CONFINED_DOMAIN: { /* confined_domain.h:7-24 */
+      REAL sx = sin(M_PI * x), cx = cos(M_PI * x), sy = sin(M_PI * y), cy = cos(M_PI * y);
+      REAL sz = sin(M_PI * z), cz = cos(M_PI * z);
+      return r4(sx * cy * cz, cx * sy * cz, -2 * cx * cy * sz, 0);
+    }
+    case IEQO_TC_ALFVEN_PERIODIC_2D: { /* alfven_periodic_2D.h:3-26 */
+      const REAL A = (REAL)0.1, al = (REAL)0.5235987755982988;
+      REAL ca = cos(al), sa = sin(al);
+      REAL phi = (REAL)6.283185307179586 * (x * ca + y * sa + t);
+      REAL cp = cos(phi), sp = sin(phi);
+      return r4(ca - A * sa * sp, sa - A * ca * sp, A * cp, 0);
+    }
+    default:
+      return zero4();
+  }
+}
+
+static real4 tc_b_boundary(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, REAL t) {
+  switch (k->c->testcase) {
+    case IEQO_TC_HALL_PERIODIC:
+    case IEQO_TC_HALL_TRAVELLING_WAVE:
+    case IEQO_TC_UNIFORM_FLOW:
+      return zero4(); /* hall_periodic.h:47-51, hall_travelling_wave.h:118-122 */
+    default:
+      return tc_b_analytical(k, ix, iy, iz, t);
+  }
+}
+
+static real4 tc_u_analytical(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, REAL t) {
+  REAL x, y, z;
+  node_xyz(k, ix, iy, iz, &x, &y, &z);
+  switch (k->c->testcase) {
+    case IEQO_TC_ROTATION_2D:
+    case IEQO_TC_GAUSSIAN_ROTATION_3D:
+      return r4(-y, x, 0, 0); /* rotation_2D.h:153-164 */
+    case IEQO_TC_ROTATION_3D: {
+      REAL sq3 = sqrt(3.0);
+      return r4((z - y) / sq3, (x - z) / sq3, (y - x) / sq3, 0);
+    }
+    case IEQO_TC_HALL_PERIODIC: {
+      const REAL n1 = (REAL)0.5773502691896258, n2 = n1, n3 = n1, a = 1, b = 1, cc = 1, al = (REAL)0.5;
+      REAL kk = (REAL)(1.0 - al * al) / al;
+      REAL u1 = a * cos(kk * y + al * kk * t * n2) + b * sin(kk * z + al * kk * t * n3);
+      REAL u2 = b * cos(kk * z + al * kk * t * n3) + cc * sin(kk * x + al * kk * t * n1);
+      REAL u3 = cc * cos(kk * x + al * kk * t * n1) + a * sin(kk * y + al * kk * t * n2);
+      return r4(u1, u2, u3, 0);
+    }
+    case IEQO_TC_HALL_TRAVELLING_WAVE: {
+      const REAL B0 = 1, Bz = 1, ux = 1, uy = 1, uz = (REAL)0.1, kk = 1, om = 1;
+      REAL xi = kk * z - om * t;
+      REAL Om = ((kk * uz - om) * (kk * uz - om) - kk * kk * Bz * Bz) / (Bz * (kk * uz - om) * kk * kk);
+      REAL fac = kk * Bz / (kk * uz - om);
+      return r4(fac * B0 * cos(Om * xi) + ux, fac * B0 * sin(Om * xi) + uy, uz, 0);
+    }
+    case IEQO_TC_CONFINED_DOMAIN:
+      return tc_b_analytical(k, ix, iy, iz, 0);
+    case IEQO_TC_ALFVEN_PERIODIC_2D: {
+      const REAL A = (REAL)0.1, al = (REAL)0.5235987755982988;
+      REAL ca = cos(al), sa = sin(al);
+      REAL phi = (REAL)6.283185307179586 * (x * ca + y * sa + t);
+      return r4(-A * sa * sin(phi), A * ca * sin(phi), A * cos(phi), 0);
+    }
+    case IEQO_TC_UNIFORM_FLOW:
+      return r4((REAL)k->c->ux, (REAL)k->c->uy, (REAL)k->c->uz, 0);
+    default:
+      return zero4();
+  }
+}
+
+/* ---- SBP derivatives: kernel/SBP_operator.cl ---- */
+static real4 diff_dir(const ctx_t *k, int dir, uint32_t ix, uint32_t iy, uint32_t iz, const real4 *f, int adj) {
+  const int w = k->op->w, c = (w - 1) / 2, nb = k->op->nb, p = k->c->periodic;
+  uint32_t i0 = dir == 0 ? ix : dir == 1 ? iy : iz;
+  uint32_t n = dir == 0 ? k->nx : dir == 1 ? k->ny : k->nz;
+  int row = bound_row(i0, n, nb, p);
+  real4 val = zero4();
+  for (int i = 0; i < w; i++) {
+    int o = i - c;
+    real4 v = f[at(k, ix, iy, iz, dir == 0 ? o : 0, dir == 1 ? o : 0, dir == 2 ? o : 0)];
+    val = add4(val, smul4(adj ? OPDADJ(k, row, i) : OPD(k, row, i), v));
+  }
+  REAL h = (REAL)(dir == 0 ? k->c->dx : dir == 1 ? k->c->dy : k->c->dz);
+  return sdiv4(val, h);
+}
+
+static REAL diff_s_dir(const ctx_t *k, int dir, uint32_t ix, uint32_t iy, uint32_t iz, const REAL *f, int adj) {
+  const int w = k->op->w, c = (w - 1) / 2, nb = k->op->nb, p = k->c->periodic;
+  uint32_t i0 = dir == 0 ? ix : dir == 1 ? iy : iz;
+  uint32_t n = dir == 0 ? k->nx : dir == 1 ? k->ny : k->nz;
+  int row = bound_row(i0, n, nb, p);
+  REAL val = 0;
+  for (int i = 0; i < w; i++) {
+    int o = i - c;
+    REAL v = f[at(k, ix, iy, iz, dir == 0 ? o : 0, dir == 1 ? o : 0, dir == 2 ? o : 0)];
+    val = val + (adj ? OPDADJ(k, row, i) : OPD(k, row, i)) * v;
+  }
+  REAL h = (REAL)(dir == 0 ? k->c->dx : dir == 1 ? k->c->dy : k->c->dz);
+  return val / h;
+}
+
+static REAL op_div(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, const real4 *f) {
+  return diff_dir(k, 0, ix, iy, iz, f, 0).x + diff_dir(k, 1, ix, iy, iz, f, 0).y + diff_dir(k, 2, ix, iy, iz, f, 0).z;
+}
+
+static real4 op_curl(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, const real4 *f) {
+  real4 v = zero4();
+  v.x = diff_dir(k, 1, ix, iy, iz, f, 0).z - diff_dir(k, 2, ix, iy, iz, f, 0).y;
+  v.y = diff_dir(k, 2, ix, iy, iz, f, 0).x - diff_dir(k, 0, ix, iy, iz, f, 0).z;
+  v.z = diff_dir(k, 0, ix, iy, iz, f, 0).y - diff_dir(k, 1, ix, iy, iz, f, 0).x;
+  return v;
+}
+
+/* ---- two-point fluxes: kernel/induction_Eq_volume.cl:13-165 ---- */
+static inline REAL comp(real4 v, int d) { return d == 0 ? v.x : d == 1 ? v.y : v.z; }
+
+static real4 num_flux(const ctx_t *k, int d, real4 Bm, real4 um, real4 Bk, real4 uk) {
+  real4 f1, f2, f3;
+  REAL bmd = comp(Bm, d), bkd = comp(Bk, d), umd = comp(um, d), ukd = comp(uk, d);
+  switch (k->c->form_uibj) {
+    case 0: f1 = smul4((REAL)0.5, add4(smul4(bmd, um), smul4(bkd, uk))); break;
+    case 1: f1 = smul4((REAL)0.25 * (bmd + bkd), add4(um, uk)); break;
+    default: f1 = smul4((REAL)0.5, add4(smul4(bkd, um), smul4(bmd, uk))); break;
+  }
+  switch (k->c->form_source) {
+    case 0: f2 = zero4(); break;
+    case 1: f2 = smul4((REAL)0.5 * (bkd - bmd), um); break;
+    default: f2 = smul4((REAL)0.25 * (bkd - bmd), add4(um, uk)); break;
+  }
+  switch (k->c->form_ujbi) {
+    case 0: f3 = smul4((REAL)0.5, add4(smul4(umd, Bm), smul4(ukd, Bk))); break;
+    case 1: f3 = smul4((REAL)0.25 * (umd + ukd), add4(Bm, Bk)); break;
+    default: f3 = smul4((REAL)0.5, add4(smul4(umd, Bk), smul4(ukd, Bm))); break;
+  }
+  return sub4(sub4(f1, f2), f3);
+}
+
+/* convec_volume: induction_Eq_volume.cl:168-212 */
+static real4 convec_volume(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, const real4 *b, const real4 *u) {
+  const int w = k->op->w, c = (w - 1) / 2, nb = k->op->nb, p = k->c->periodic;
+  uint32_t idx = calc_idx(k, ix, iy, iz);
+  int rows[3] = {bound_row(ix, k->nx, nb, p), bound_row(iy, k->ny, nb, p), bound_row(iz, k->nz, nb, p)};
+  real4 Bm = b[idx], um = u[idx], val = zero4();
+  for (int d = 0; d < 3; d++) {
+    real4 dB = zero4();
+    for (int i = 0; i < w; i++) {
+      int o = i - c;
+      uint32_t j = at(k, ix, iy, iz, d == 0 ? o : 0, d == 1 ? o : 0, d == 2 ? o : 0);
+      dB = add4(dB, smul4(OPD(k, rows[d], i), num_flux(k, d, Bm, um, b[j], u[j])));
+    }
+    double h = d == 0 ? k->c->dx : d == 1 ? k->c->dy : k->c->dz;
+    val = add4(val, smul4((REAL)(2.0 / h), dB));
+  }
+  return val;
+}
+
+/* Hall_volume: induction_Eq_volume.cl:215-256 */
+static real4 hall_volume(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, const real4 *b, const real4 *J) {
+  const int w = k->op->w, c = (w - 1) / 2, nb = k->op->nb, p = k->c->periodic;
+  int rows[3] = {bound_row(ix, k->nx, nb, p), bound_row(iy, k->ny, nb, p), bound_row(iz, k->nz, nb, p)};
+  real4 val = zero4();
+  for (int d = 0; d < 3; d++) {
+    real4 dB = zero4();
+    for (int i = 0; i < w; i++) {
+      int o = i - c;
+      uint32_t j = at(k, ix, iy, iz, d == 0 ? o : 0, d == 1 ? o : 0, d == 2 ? o : 0);
+      real4 B = b[j], Jk = J[j];
+      dB = add4(dB, smul4(OPD(k, rows[d], i), sub4(smul4(comp(Jk, d), B), smul4(comp(B, d), Jk))));
+    }
+    double h = d == 0 ? k->c->dx : d == 1 ? k->c->dy : k->c->dz;
+    val = add4(val, smul4((REAL)(1.0 / h), dB));
+  }
+  return val;
+}
+
+/* convec_surface: induction_Eq_surface.cl:7-33 */
+static real4 convec_surface(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, const real4 *b, const real4 *u, REAL t) {
+  if (k->c->periodic) return zero4();
+  uint32_t idx = calc_idx(k, ix, iy, iz);
+  real4 g = tc_b_boundary(k, ix, iy, iz, t);
+  real4 U = u[idx], diff = sub4(b[idx], g);
+  REAL m0 = (REAL)k->op->minv[0];
+  REAL sx = (REAL)((ix == 0) * (U.x > 0)) * (REAL)(-1) + (REAL)((ix == k->nx - 1) * (U.x < 0)) * (REAL)1;
+  REAL sy = (REAL)((iy == 0) * (U.y > 0)) * (REAL)(-1) + (REAL)((iy == k->ny - 1) * (U.y < 0)) * (REAL)1;
+  REAL sz = (REAL)((iz == 0) * (U.z > 0)) * (REAL)(-1) + (REAL)((iz == k->nz - 1) * (U.z < 0)) * (REAL)1;
+  real4 vx = smul4((REAL)(m0 / (REAL)k->c->dx) * sx * U.x, diff);
+  real4 vy = smul4((REAL)(m0 / (REAL)k->c->dy) * sy * U.y, diff);
+  real4 vz = smul4((REAL)(m0 / (REAL)k->c->dz) * sz * U.z, diff);
+  return add4(add4(vx, vy), vz);
+}
+
+/* convec_Hall_surface: induction_Eq_surface.cl:35-81 */
+static real4 hall_surface(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, const real4 *b, const real4 *J,
+                          const real4 *u) {
+  if (k->c->periodic) return zero4();
+  uint32_t idx = calc_idx(k, ix, iy, iz);
+  real4 B = b[idx], Jc = J[idx];
+  real4 tv = sub4(smul4((REAL)0.5, u[idx]), Jc);
+  REAL m0 = (REAL)k->op->minv[0];
+  uint32_t ii[3] = {ix, iy, iz}, nn[3] = {k->nx, k->ny, k->nz};
+  double hh[3] = {k->c->dx, k->c->dy, k->c->dz};
+  real4 val = zero4();
+  for (int d = 0; d < 3; d++) {
+    int l = ii[d] == 0, r = ii[d] == nn[d] - 1;
+    REAL tvd = comp(tv, d);
+    REAL s1 = (REAL)(l * (tvd > 0)) * (REAL)(-1) + (REAL)(r * (tvd < 0)) * (REAL)1;
+    REAL s2 = (REAL)l * (REAL)(-1) + (REAL)r * (REAL)1;
+    real4 t = add4(smul4(s1 * tvd, B), smul4(s2 * comp(B, d), Jc));
+    val = add4(val, smul4((REAL)(m0 / (REAL)hh[d]), t));
+  }
+  return val;
+}
+
+/* diff_HOD_*: include/artificial_dissipation.h:58-101 */
+static real4 diff_hod(const ctx_t *k, int d, uint32_t ix, uint32_t iy, uint32_t iz, int row, int start, const real4 *f) {
+  const int wh = k->op->w_hod, nb = k->op->nb;
+  real4 hod = zero4();
+  REAL sgn = ((k->op->order / 2 + 1) % 2) ? (REAL)-1 : (REAL)1; /* pown(-1.0, ORDER/2+1) */
+  for (int i = 0; i < wh; i++) {
+    int o = i + start;
+    REAL coef = (REAL)k->op->minv[row] * (REAL)k->op->dho[row * wh + start + wh - 1] * (REAL)k->op->dho[nb * wh + i] * sgn;
+    hod = add4(hod, smul4(coef, f[at(k, ix, iy, iz, d == 0 ? o : 0, d == 1 ? o : 0, d == 2 ? o : 0)]));
+  }
+  return hod;
+}
+
+/* high_order_dissipation: kernel/artificial_dissipation.cl:13-32 */
+static real4 ho_dissipation(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, const real4 *b) {
+  const int nb = k->op->nb, wh = k->op->w_hod, p = k->c->periodic;
+  int rows[3] = {bound_row(ix, k->nx, nb, p), bound_row(iy, k->ny, nb, p), bound_row(iz, k->nz, nb, p)};
+  REAL F = (REAL)k->c->ho_factor;
+  real4 h = zero4();
+  for (int s = -wh + 1; s < 1; s++) {
+    h = add4(add4(add4(h, smul4(F, diff_hod(k, 0, ix, iy, iz, rows[0], s, b))), smul4(F, diff_hod(k, 1, ix, iy, iz, rows[1], s, b))),
+             smul4(F, diff_hod(k, 2, ix, iy, iz, rows[2], s, b)));
+  }
+  h.w = 0;
+  return h;
+}
+
+static inline real4 sigma4(real4 a, real4 b) { return fmax4(fabs4(a), fabs4(b)); } /* artificial_dissipation.h:51-53 */
+
+/* kappa4: include/artificial_dissipation.h:23-45; k is the reference's uint argument */
+static real4 kappa4(const ctx_t *kc, uint32_t k, REAL dx, real4 b1, real4 b2, real4 u1, real4 u2) {
+  real4 db = sub4(b1, b2);
+  real4 s = mul4(mul4(fmax4(fabs4(u1), fabs4(u2)), db), db);
+  k = k + (k == 0);
+  REAL kk = (REAL)(uint32_t)(k * k);
+  REAL lo = kk * (REAL)kc->c->mp2min * (REAL)kc->c->cmin * dx;
+  REAL hi = kk * (REAL)kc->c->mp2max * (REAL)kc->c->cmax * dx;
+  REAL sv[3] = {s.x, s.y, s.z}, out[3];
+  for (int q = 0; q < 3; q++)
+    out[q] = (REAL)((lo < sv[q]) && (sv[q] < hi)) * sqrt(fabs(sv[q] - lo) / fabs(hi - lo)) + (REAL)(sv[q] >= hi);
+  return r4(out[0], out[1], out[2], 0);
+}
+
+/* first_order_dissipation: kernel/artificial_dissipation.cl:36-79 */
+static real4 fo_dissipation(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, const real4 *b, const real4 *u) {
+  const int w = k->op->w, c = (w - 1) / 2, nb = k->op->nb, p = k->c->periodic;
+  int rows[3] = {bound_row(ix, k->nx, nb, p), bound_row(iy, k->ny, nb, p), bound_row(iz, k->nz, nb, p)};
+  uint32_t i0 = calc_idx(k, ix, iy, iz);
+  real4 fo = zero4();
+  for (int d = 0; d < 3; d++) {
+    real4 tmp = zero4();
+    for (int i = 0; i < w; i++) {
+      int o = i - c;
+      uint32_t j = at(k, ix, iy, iz, d == 0 ? o : 0, d == 1 ? o : 0, d == 2 ? o : 0);
+      real4 sg = sigma4(u[j], u[i0]);
+      real4 t = smul4((REAL)fabs(OPD(k, rows[d], i)), mul4(sg, sub4(b[j], b[i0])));
+      tmp.x += t.x; tmp.y += t.y; tmp.z += t.z;
+    }
+    REAL h = (REAL)(d == 0 ? k->c->dx : d == 1 ? k->c->dy : k->c->dz);
+    fo = add4(fo, sdiv4(tmp, h));
+  }
+  return fo;
+}
+
+/* adaptive_dissipation: kernel/artificial_dissipation.cl:83-160 */
+static real4 ad_dissipation(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, const real4 *b, const real4 *u) {
+  const int w = k->op->w, c = (w - 1) / 2, nb = k->op->nb, wh = k->op->w_hod, p = k->c->periodic;
+  int rows[3] = {bound_row(ix, k->nx, nb, p), bound_row(iy, k->ny, nb, p), bound_row(iz, k->nz, nb, p)};
+  uint32_t i0 = calc_idx(k, ix, iy, iz);
+  real4 ad = zero4();
+  for (int d = 0; d < 3; d++) {
+    REAL h = (REAL)(d == 0 ? k->c->dx : d == 1 ? k->c->dy : k->c->dz);
+    real4 tmp = zero4();
+    for (int i = 0; i < w; i++) {
+      int o = i - c;
+      uint32_t j = at(k, ix, iy, iz, d == 0 ? o : 0, d == 1 ? o : 0, d == 2 ? o : 0);
+      uint32_t kk = (uint32_t)i - (uint32_t)c; /* abs() of a uint: unchanged */
+      real4 kap = kappa4(k, kk, h, b[j], b[i0], u[j], u[i0]);
+      real4 sg = sigma4(u[j], u[i0]);
+      real4 t = smul4((REAL)fabs(OPD(k, rows[d], i)), mul4(mul4(kap, sg), sub4(b[j], b[i0])));
+      tmp.x += t.x; tmp.y += t.y; tmp.z += t.z;
+    }
+    ad.x += tmp.x / h; ad.y += tmp.y / h; ad.z += tmp.z / h;
+  }
+  REAL F = (REAL)k->c->ho_factor;
+  for (int term = -wh + 1; term <= 0; term++) {
+    real4 acc = zero4();
+    for (int d = 0; d < 3; d++) {
+      REAL h = (REAL)(d == 0 ? k->c->dx : d == 1 ? k->c->dy : k->c->dz);
+      int o1 = term + wh - 1, o2 = term + wh - 2;
+      uint32_t j1 = at(k, ix, iy, iz, d == 0 ? o1 : 0, d == 1 ? o1 : 0, d == 2 ? o1 : 0);
+      uint32_t j2 = at(k, ix, iy, iz, d == 0 ? o2 : 0, d == 1 ? o2 : 0, d == 2 ? o2 : 0);
+      real4 kap = kappa4(k, 1, h, b[j1], b[j2], u[j1], u[j2]);
+      real4 hod = diff_hod(k, d, ix, iy, iz, rows[d], term, b);
+      acc.x += (1 - kap.x) * F * hod.x;
+      acc.y += (1 - kap.y) * F * hod.y;
+      acc.z += (1 - kap.z) * F * hod.z;
+    }
+    ad.x += acc.x; ad.y += acc.y; ad.z += acc.z;
+  }
+  ad.w = 0;
+  return ad;
+}
+
+/* DB: kernel_time_integrator.cl:9-31 */
+static real4 DB(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, const real4 *b, const real4 *J, const real4 *u, REAL t) {
+  real4 db = convec_volume(k, ix, iy, iz, b, u);
+  if (k->c->hall)
+    db = add4(add4(db, hall_surface(k, ix, iy, iz, b, J, u)), hall_volume(k, ix, iy, iz, b, J));
+  else
+    db = add4(db, convec_surface(k, ix, iy, iz, b, u, t));
+  switch (k->c->dissipation) {
+    case 1: db = add4(db, ho_dissipation(k, ix, iy, iz, b)); break;
+    case 2: db = add4(db, fo_dissipation(k, ix, iy, iz, b, u)); break;
+    case 3: db = add4(db, ad_dissipation(k, ix, iy, iz, b, u)); break;
+    default: break;
+  }
+  return db;
+}
+
+/* ---- RK stage table: kernel_time_integrator.cl:224-1516 ----
+ * kind 1: out = P + s*DB(Y)                 kind 2: out = a*P + b*(Q + s*DB(Y))
+ * kind 3: out = s*DB(Y)                     kind 4: out = a*P + s*DB(Y)
+ * kind 5: 2R update  X = X + a*D; O = X + b*D (time)   kind 6: 2N update X = X + b*D (time)
+ * kind 7: SSPRK104_06 pointwise pair (time)  kind 8: time-only stage
+ * kind 9: calc_time   kind 10: calc_u   kind 11: calc_curlB_rho
+ * Field slots: 0 = b1, 1 = b2, 2 = b3 (or db).  tc < 0: no time update. */
+typedef struct {
+  const char *name;
+  int nf;       /* 2 or 3 field variant */
+  int kind;
+  int out, P, Q, Y; /* field slots */
+  double a, b, s_dt; /* coefficients; s = s_dt * DT */
+  double tc;         /* time[1] = time[0] + tc*DT when >= 0 */
+} stage_t;
+
+#define CK_B1 (1432997174477.0 / 9575080441755.0)
+static const stage_t STAGES[] = {
+  {"SSPRK33_1", 3, 1, 1, 0, -1, 0, 0, 0, 1.0, -1},
+  {"SSPRK33_2a", 3, 8, -1, -1, -1, -1, 0, 0, 0, 1.0},
+  {"SSPRK33_2b", 3, 2, 2, 0, 1, 1, 0.75, 0.25, 1.0, -1},
+  {"SSPRK33_3a", 3, 8, -1, -1, -1, -1, 0, 0, 0, 0.5},
+  {"SSPRK33_3b", 3, 2, 0, 0, 2, 2, 1.0 / 3.0, 2.0 / 3.0, 1.0, -1},
+  {"SSPRK104_01", 3, 1, 1, 0, -1, 0, 0, 0, 1.0 / 6.0, -1},
+  {"SSPRK104_02a", 3, 8, -1, -1, -1, -1, 0, 0, 0, 1.0 / 6.0},
+  {"SSPRK104_02b", 3, 1, 2, 1, -1, 1, 0, 0, 1.0 / 6.0, -1},
+  {"SSPRK104_03a", 3, 8, -1, -1, -1, -1, 0, 0, 0, 1.0 / 3.0},
+  {"SSPRK104_03b", 3, 1, 1, 2, -1, 2, 0, 0, 1.0 / 6.0, -1},
+  {"SSPRK104_04a", 3, 8, -1, -1, -1, -1, 0, 0, 0, 0.5},
+  {"SSPRK104_04b", 3, 1, 2, 1, -1, 1, 0, 0, 1.0 / 6.0, -1},
+  {"SSPRK104_05a", 3, 8, -1, -1, -1, -1, 0, 0, 0, 2.0 / 3.0},
+  {"SSPRK104_05b", 3, 1, 1, 2, -1, 2, 0, 0, 1.0 / 6.0, -1},
+  {"SSPRK104_06", 3, 7, -1, -1, -1, -1, 0, 0, 0, 1.0 / 3.0},
+  {"SSPRK104_07", 3, 1, 0, 1, -1, 1, 0, 0, 1.0 / 6.0, -1},
+  {"SSPRK104_08a", 3, 8, -1, -1, -1, -1, 0, 0, 0, 0.5},
+  {"SSPRK104_08b", 3, 1, 1, 0, -1, 0, 0, 0, 1.0 / 6.0, -1},
+  {"SSPRK104_09a", 3, 8, -1, -1, -1, -1, 0, 0, 0, 2.0 / 3.0},
+  {"SSPRK104_09b", 3, 1, 0, 1, -1, 1, 0, 0, 1.0 / 6.0, -1},
+  {"SSPRK104_10a", 3, 8, -1, -1, -1, -1, 0, 0, 0, 5.0 / 6.0},
+  {"SSPRK104_10b", 3, 1, 1, 0, -1, 0, 0, 0, 1.0 / 6.0, -1},
+  {"SSPRK104_11a", 3, 8, -1, -1, -1, -1, 0, 0, 0, 1.0},
+  {"SSPRK104_11b", 3, 2, 0, 2, 1, 1, 1.0, 0.6, 1.0 / 6.0, -1},
+  /* KennedyCarpenterLewis2R54C: slots b1=0, b2=1, db=2 */
+  {"KennedyCarpenterLewis2R54C_1a", 3, 3, 2, -1, -1, 0, 0, 0, 1.0, -1},
+  {"KennedyCarpenterLewis2R54C_1b", 3, 5, 1, 0, 2, -1, 0.22502245872571303, -0.17379315208537388, 0, 0.22502245872571303},
+  {"KennedyCarpenterLewis2R54C_2a", 3, 3, 2, -1, -1, 0, 0, 0, 1.0, -1},
+  {"KennedyCarpenterLewis2R54C_2b", 3, 5, 0, 1, 2, -1, 0.5440433129514047, -0.1630884872250029, 0, 0.5952726195917439},
+  {"KennedyCarpenterLewis2R54C_3a", 3, 3, 2, -1, -1, 1, 0, 0, 1.0, -1},
+  {"KennedyCarpenterLewis2R54C_3b", 3, 5, 1, 0, 2, -1, 0.14456824349399464, -0.5179208398863779, 0, -1},
+  {"KennedyCarpenterLewis2R54C_4a", 3, 3, 2, -1, -1, 0, 0, 0, 1.0, -1},
+  {"KennedyCarpenterLewis2R54C_4b", 3, 5, 0, 1, 2, -1, 0.7866643421983568, -0.19416305717199442, 0, 0.8454958781727144},
+  {"KennedyCarpenterLewis2R54C_5", 3, 1, 0, 0, -1, 1, 0, 0, 0.34866717899927996, -1},
+  /* CalvoFrancoRandez2R64: b-update is X = X + B*D; O = X + A*D */
+  {"CalvoFrancoRandez2R64_1a", 3, 3, 2, -1, -1, 0, 0, 0, 1.0, -1},
+  {"CalvoFrancoRandez2R64_1b", 3, 5, 1, 0, 2, -1, 0.10893125722541, 0.17985400977138, 0, 0.28878526699679},
+  {"CalvoFrancoRandez2R64_2a", 3, 3, 2, -1, -1, 1, 0, 0, 1.0, -1},
+  {"CalvoFrancoRandez2R64_2b", 3, 5, 1, 0, 2, -1, 0.13201701492152, 0.14081893152111, 0, 0.38176720366804},
+  {"CalvoFrancoRandez2R64_3a", 3, 3, 2, -1, -1, 1, 0, 0, 1.0, -1},
+  {"CalvoFrancoRandez2R64_3b", 3, 5, 1, 0, 2, -1, 0.38911623225517, 0.08255631629428, 0, 0.71262082069639},
+  {"CalvoFrancoRandez2R64_4a", 3, 3, 2, -1, -1, 1, 0, 0, 1.0, -1},
+  {"CalvoFrancoRandez2R64_4b", 3, 5, 1, 0, 2, -1, -0.59203884581148, 0.65804425034331, 0, 0.69606990893393},
+  {"CalvoFrancoRandez2R64_5a", 3, 3, 2, -1, -1, 1, 0, 0, 1.0, -1},
+  {"CalvoFrancoRandez2R64_5b", 3, 5, 1, 0, 2, -1, 0.47385028714844, 0.31862993413251, 0, 0.83050587987157},
+  {"CalvoFrancoRandez2R64_6", 3, 1, 0, 0, -1, 1, 0, 0, 0.48812405426094, -1},
+  /* CarpenterKennedy2N54: slots b1=0, b2=1 */
+  {"CarpenterKennedy2N54_1a", 2, 3, 1, -1, -1, 0, 0, 0, 1.0, -1},
+  {"CarpenterKennedy2N54_1b", 2, 6, 0, 1, -1, -1, 0, CK_B1, 0, CK_B1},
+  {"CarpenterKennedy2N54_2a", 2, 4, 1, 1, -1, 0, -567301805773.0 / 1357537059087.0, 0, 1.0, -1},
+  {"CarpenterKennedy2N54_2b", 2, 6, 0, 1, -1, -1, 0, 5161836677717.0 / 13612068292357.0, 0, 2526269341429.0 / 6820363962896.0},
+  {"CarpenterKennedy2N54_3a", 2, 4, 1, 1, -1, 0, -2404267990393.0 / 2016746695238.0, 0, 1.0, -1},
+  {"CarpenterKennedy2N54_3b", 2, 6, 0, 1, -1, -1, 0, 1720146321549.0 / 2090206949498.0, 0, 2006345519317.0 / 3224310063776.0},
+  {"CarpenterKennedy2N54_4a", 2, 4, 1, 1, -1, 0, -3550918686646.0 / 2091501179385.0, 0, 1.0, -1},
+  {"CarpenterKennedy2N54_4b", 2, 6, 0, 1, -1, -1, 0, 3134564353537.0 / 4481467310338.0, 0, 2802321613138.0 / 2924317926251.0},
+  {"CarpenterKennedy2N54_5a", 2, 4, 1, 1, -1, 0, -1275806237668.0 / 842570457699.0, 0, 1.0, -1},
+  {"CarpenterKennedy2N54_5b", 2, 6, 0, 1, -1, -1, 0, 2277821191437.0 / 14882151754819.0, 0, -1},
+  /* ToulorgeDesmet2N84F */
+  {"ToulorgeDesmet2N84F_1a", 2, 3, 1, -1, -1, 0, 0, 0, 1.0, -1},
+  {"ToulorgeDesmet2N84F_1b", 2, 6, 0, 1, -1, -1, 0, 0.08037936882736950, 0, 0.08037936882736950},
+  {"ToulorgeDesmet2N84F_2a", 2, 4, 1, 1, -1, 0, -0.5534431294501569, 0, 1.0, -1},
+  {"ToulorgeDesmet2N84F_2b", 2, 6, 0, 1, -1, -1, 0, 0.5388497458569843, 0, 0.3210064250338430},
+  {"ToulorgeDesmet2N84F_3a", 2, 4, 1, 1, -1, 0, 0.01065987570203490, 0, 1.0, -1},
+  {"ToulorgeDesmet2N84F_3b", 2, 6, 0, 1, -1, -1, 0, 0.01974974409031960, 0, 0.3408501826604660},
+  {"ToulorgeDesmet2N84F_4a", 2, 4, 1, 1, -1, 0, -0.5515812888932000, 0, 1.0, -1},
+  {"ToulorgeDesmet2N84F_4b", 2, 6, 0, 1, -1, -1, 0, 0.09911841297339970, 0, 0.3850364824285470},
+  {"ToulorgeDesmet2N84F_5a", 2, 4, 1, 1, -1, 0, -1.885790377558741, 0, 1.0, -1},
+  {"ToulorgeDesmet2N84F_5b", 2, 6, 0, 1, -1, -1, 0, 0.7466920411064123, 0, 0.5040052477534100},
+  {"ToulorgeDesmet2N84F_6a", 2, 4, 1, 1, -1, 0, -5.701295742793264, 0, 1.0, -1},
+  {"ToulorgeDesmet2N84F_6b", 2, 6, 0, 1, -1, -1, 0, 1.679584245618894, 0, 0.6578977561168540},
+  {"ToulorgeDesmet2N84F_7a", 2, 4, 1, 1, -1, 0, 2.113903965664793, 0, 1.0, -1},
+  {"ToulorgeDesmet2N84F_7b", 2, 6, 0, 1, -1, -1, 0, 0.2433728067008188, 0, 0.9484087623348481},
+  {"ToulorgeDesmet2N84F_8a", 2, 4, 1, 1, -1, 0, -0.5339578826675280, 0, 1.0, -1},
+  {"ToulorgeDesmet2N84F_8b", 2, 6, 0, 1, -1, -1, 0, 0.1422730459001373, 0, -1},
+  /* auxiliary kernels: kernel_time_integrator.cl:42-208 */
+  {"calc_time_2_args", 2, 9, -1, -1, -1, -1, 0, 0, 0, -1},
+  {"calc_time_3_args", 3, 9, -1, -1, -1, -1, 0, 0, 0, -1},
+  {"calc_u_2_args", 2, 10, -1, -1, -1, -1, 0, 0, 0, -1},
+  {"calc_u_3_args", 3, 10, -1, -1, -1, -1, 0, 0, 0, -1},
+  {"calc_curlB_rho_1_2args", 2, 11, -1, -1, -1, 0, 0, 0, 0, -1},
+  {"calc_curlB_rho_2_2args", 2, 11, -1, -1, -1, 1, 0, 0, 0, -1},
+  {"calc_curlB_rho_1_3args", 3, 11, -1, -1, -1, 0, 0, 0, 0, -1},
+  {"calc_curlB_rho_2_3args", 3, 11, -1, -1, -1, 1, 0, 0, 0, -1},
+  {"calc_curlB_rho_3_3args", 3, 11, -1, -1, -1, 2, 0, 0, 0, -1},
+};
+#define NSTAGES ((int)(sizeof(STAGES) / sizeof(STAGES[0])))
+
+static void set_threads(const ieqo_cfg *c) {
+#ifdef _OPENMP
+  if (c->nthreads > 0) omp_set_num_threads(c->nthreads);
+#else
+  (void)c;
+#endif
+}
+
+static int run_stage(const ctx_t *k, const stage_t *st, const uint32_t g[3], void **args) {
+  real4 *F[3] = {(real4 *)args[0], (real4 *)args[1], st->nf == 3 ? (real4 *)args[2] : 0};
+  int o = st->nf;
+  real4 *J = (real4 *)args[o], *u = (real4 *)args[o + 1];
+  REAL *rho = (REAL *)args[o + 2], *time = (REAL *)args[o + 3];
+  const REAL DT = (REAL)k->c->dt;
+  REAL t1 = time[1];
+  if (st->kind == 9) { /* calc_time_*_args */
+    time[0] = time[0] + DT;
+    time[1] = time[0];
+    return 0;
+  }
+  if (st->kind == 8) {
+    time[1] = time[0] + (REAL)(st->tc * k->c->dt);
+    return 0;
+  }
+  /* NDRange g over nodes; the idx == 0 work-item writes time (kernel_time_integrator.cl:250-254). */
+  int64_t nz = g[2], ny = g[1], nx = g[0];
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int64_t iz = 0; iz < nz; iz++)
+    for (int64_t iy = 0; iy < ny; iy++)
+      for (int64_t ix = 0; ix < nx; ix++) {
+        uint32_t idx = calc_idx(k, (uint32_t)ix, (uint32_t)iy, (uint32_t)iz);
+        switch (st->kind) {
+          case 1: { /* out = P + s*DB(Y) */
+            REAL s = (REAL)(st->s_dt * k->c->dt);
+            real4 d = DB(k, ix, iy, iz, F[st->Y], J, u, t1);
+            F[st->out][idx] = add4(F[st->P][idx], smul4(s, d));
+          } break;
+          case 2: { /* out = a*P + b*(Q + s*DB(Y)) */
+            real4 d = DB(k, ix, iy, iz, F[st->Y], J, u, t1);
+            REAL s = (REAL)(st->s_dt * k->c->dt);
+            F[st->out][idx] = add4(smul4((REAL)st->a, F[st->P][idx]), smul4((REAL)st->b, add4(F[st->Q][idx], smul4(s, d))));
+          } break;
+          case 3: { /* out = DT*DB(Y) */
+            real4 d = DB(k, ix, iy, iz, F[st->Y], J, u, t1);
+            F[st->out][idx] = smul4(DT, d);
+          } break;
+          case 4: { /* out = a*P + DT*DB(Y) */
+            real4 d = DB(k, ix, iy, iz, F[st->Y], J, u, t1);
+            F[st->out][idx] = add4(smul4((REAL)st->a, F[st->P][idx]), smul4(DT, d));
+          } break;
+          case 5: { /* P = P + a*D; out = P + b*D (2R b-stages) */
+            real4 D = F[st->Q][idx];
+            F[st->P][idx] = add4(F[st->P][idx], smul4((REAL)st->a, D));
+            F[st->out][idx] = add4(F[st->P][idx], smul4((REAL)st->b, D));
+          } break;
+          case 6: /* b1 = b1 + B*b2 (2N b-stages) */
+            F[st->out][idx] = add4(F[st->out][idx], smul4((REAL)st->b, F[st->P][idx]));
+            break;
+          case 7: { /* SSPRK104_06 */
+            real4 b1 = F[0][idx], b2 = F[1][idx];
+            F[2][idx] = add4(smul4((REAL)0.04, b1), smul4((REAL)0.36, b2));
+            F[1][idx] = add4(smul4((REAL)0.60, b1), smul4((REAL)0.40, b2));
+          } break;
+          case 10: /* calc_u */
+            u[idx] = tc_u_analytical(k, ix, iy, iz, t1);
+            break;
+          case 11: /* calc_curlB_rho */
+            J[idx] = sdiv4(op_curl(k, ix, iy, iz, F[st->Y]), rho[idx]);
+            break;
+        }
+      }
+  if (st->tc >= 0 && (st->kind == 5 || st->kind == 6 || st->kind == 7)) {
+    if (st->kind == 5 || st->kind == 6)
+      time[1] = time[0] + (REAL)st->tc * DT; /* coef_C * (REAL)(DT) */
+    else
+      time[1] = time[0] + (REAL)(st->tc * k->c->dt);
+  }
+  return 0;
+}
+
+/* Work-group tree reduction over group g (kernel_operator.cl:31-46, kernel_dot_product.cl:56-71). */
+static REAL tree_sum(REAL *part, int gs) {
+  for (int i = gs / 2; i > 0; i /= 2)
+    for (int l = 0; l < i; l++) part[l] += part[l + i];
+  return part[0];
+}
+
+static REAL norm_fac(const ctx_t *k, uint32_t gid) {
+  /* calc_sub_idx: utils.h:72-83 */
+  uint32_t NXY = k->nx * k->ny;
+  uint32_t iz = gid / NXY, iy = (gid - iz * NXY) / k->nx, ix = gid - iy * k->nx - iz * k->ny * k->nx;
+  const int nb = k->op->nb, p = k->c->periodic;
+  int bx = bound_row(ix, k->nx, nb, p), by = bound_row(iy, k->ny, nb, p), bz = bound_row(iz, k->nz, nb, p);
+  return ((REAL)k->c->dx / (REAL)k->op->minv[bx]) * ((REAL)k->c->dy / (REAL)k->op->minv[by]) *
+         ((REAL)k->c->dz / (REAL)k->op->minv[bz]);
+}
+
+/* reductions: norm2/norm2_diff/norm2_S (kernel_operator.cl:11-137), dot_product (kernel_dot_product.cl:47-74) */
+static int run_reduction(const ctx_t *k, int which, uint32_t gsize, uint32_t lsize, void **args) {
+  if (lsize == 0 || gsize % lsize) return -3;
+  uint32_t ngroups = gsize / lsize;
+  REAL *out = (REAL *)args[which == 1 ? 2 : which == 3 ? 2 : 1];
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < ngroups; g++) {
+    REAL part[4096];
+    for (uint32_t l = 0; l < lsize; l++) {
+      uint32_t gid = (uint32_t)g * lsize + l;
+      REAL v;
+      switch (which) {
+        case 0: { real4 f = ((real4 *)args[0])[gid]; v = norm_fac(k, gid) * dot4(f, f); } break;
+        case 1: { real4 d = sub4(((real4 *)args[0])[gid], ((real4 *)args[1])[gid]); v = norm_fac(k, gid) * dot4(d, d); } break;
+        case 2: { REAL f = ((REAL *)args[0])[gid]; v = norm_fac(k, gid) * f * f; } break;
+        default: v = ((REAL *)args[0])[gid] * ((REAL *)args[1])[gid]; break;
+      }
+      part[l] = v;
+    }
+    out[g] = tree_sum(part, (int)lsize);
+  }
+  return 0;
+}
+
+/* -Laplacian for divergence cleaning: kernel_div_cleaning.cl:11-56 */
+static REAL laplace_dc(const ctx_t *k, uint32_t ix, uint32_t iy, uint32_t iz, const REAL *f) {
+  int L = k->c->laplace;
+  const double *tab = k->op->lap[L];
+  int nbl = k->op->lap_nb[L], wl = k->op->lap_w[L], c = (wl - 1) / 2, p = k->c->periodic;
+  int bx = bound_row(ix, k->nx, nbl, p), by = bound_row(iy, k->ny, nbl, p), bz = bound_row(iz, k->nz, nbl, p);
+  REAL dx = (REAL)k->c->dx, dy = (REAL)k->c->dy, dz = (REAL)k->c->dz;
+  REAL val = 0;
+  for (int i = 0; i < wl; i++) {
+    int o = i - c;
+    val = val + (REAL)tab[bx * wl + i] * f[at(k, ix, iy, iz, o, 0, 0)] / (dx * dx) +
+          (REAL)tab[by * wl + i] * f[at(k, ix, iy, iz, 0, o, 0)] / (dy * dy) +
+          (REAL)tab[bz * wl + i] * f[at(k, ix, iy, iz, 0, 0, o)] / (dz * dz);
+  }
+  return val;
+}
+
+#define FOR_RANGE(off)                                                     \
+  _Pragma("omp parallel for collapse(2) schedule(static)")                 \
+  for (int64_t jz = 0; jz < (int64_t)g[2]; jz++)                           \
+    for (int64_t jy = 0; jy < (int64_t)g[1]; jy++)                         \
+      for (int64_t jx = 0; jx < (int64_t)g[0]; jx++) {                     \
+        uint32_t ix = (uint32_t)jx + (off), iy = (uint32_t)jy + (off), iz = (uint32_t)jz + (off); \
+        uint32_t idx = calc_idx(k, ix, iy, iz);
+
+#define END_RANGE }
+
+static int run_other(const ctx_t *k, const char *name, const uint32_t g[3], const uint32_t l[3], void **args, int nargs) {
+  uint32_t B = (uint32_t)k->c->bnodes;
+  if (!strcmp(name, "init_b") || !strcmp(name, "init_u") || !strcmp(name, "init_rho") || !strcmp(name, "analytical_b")) {
+    /* kernel_init.cl:3-48 */
+    int which = !strcmp(name, "init_b") ? 0 : !strcmp(name, "init_u") ? 1 : !strcmp(name, "init_rho") ? 2 : 3;
+    if (nargs < (which == 3 ? 2 : 1)) return -2;
+    REAL t = which == 3 ? ((REAL *)args[1])[0] : 0;
+    FOR_RANGE(0)
+      if (which == 0) ((real4 *)args[0])[idx] = tc_b_analytical(k, ix, iy, iz, 0);
+      else if (which == 1) ((real4 *)args[0])[idx] = tc_u_analytical(k, ix, iy, iz, 0);
+      else if (which == 2) ((REAL *)args[0])[idx] = (REAL)1;
+      else ((real4 *)args[0])[idx] = tc_b_analytical(k, ix, iy, iz, t);
+    END_RANGE
+    return 0;
+  }
+  if (!strcmp(name, "norm2")) return nargs < 2 ? -2 : run_reduction(k, 0, g[0], l[0], args);
+  if (!strcmp(name, "norm2_diff")) return nargs < 3 ? -2 : run_reduction(k, 1, g[0], l[0], args);
+  if (!strcmp(name, "norm2_S")) return nargs < 2 ? -2 : run_reduction(k, 2, g[0], l[0], args);
+  if (!strcmp(name, "dot_product")) return nargs < 3 ? -2 : run_reduction(k, 3, g[0], l[0], args);
+  if (!strcmp(name, "calc_div")) { /* kernel_operator.cl:145-155 */
+    if (nargs < 2) return -2;
+    FOR_RANGE(0) ((REAL *)args[1])[idx] = op_div(k, ix, iy, iz, (real4 *)args[0]); END_RANGE
+    return 0;
+  }
+  if (!strcmp(name, "calc_curl")) { /* kernel_operator.cl:158-167 */
+    if (nargs < 2) return -2;
+    FOR_RANGE(0) ((real4 *)args[1])[idx] = op_curl(k, ix, iy, iz, (real4 *)args[0]); END_RANGE
+    return 0;
+  }
+  if (!strcmp(name, "calc_grad")) { /* kernel_operator.cl:170-179, grad: SBP_operator.cl:233-243 */
+    if (nargs < 2) return -2;
+    FOR_RANGE(0)
+      const REAL *s = (REAL *)args[0];
+      ((real4 *)args[1])[idx] = r4(diff_s_dir(k, 0, ix, iy, iz, s, 0), diff_s_dir(k, 1, ix, iy, iz, s, 0), diff_s_dir(k, 2, ix, iy, iz, s, 0), 0);
+    END_RANGE
+    return 0;
+  }
+  if (!strcmp(name, "calc_div_divcleaning")) { /* kernel_div_cleaning.cl:60-70 */
+    if (nargs < 2) return -2;
+    FOR_RANGE(B) ((REAL *)args[1])[idx] = op_div(k, ix, iy, iz, (real4 *)args[0]); END_RANGE
+    return 0;
+  }
+  if (!strcmp(name, "calc_laplace_divcleaning")) { /* kernel_div_cleaning.cl:75-84 */
+    if (nargs < 2) return -2;
+    FOR_RANGE(B) ((REAL *)args[1])[idx] = laplace_dc(k, ix, iy, iz, (REAL *)args[0]); END_RANGE
+    return 0;
+  }
+  if (!strcmp(name, "projector_divcleaning")) { /* kernel_div_cleaning.cl:96-109 */
+    if (nargs < 2) return -2;
+    real4 *b = (real4 *)args[0];
+    const REAL *phi = (REAL *)args[1];
+    int lns = k->c->laplace == IEQ_LAP_WS_LN;
+    FOR_RANGE(0)
+      REAL gx = diff_s_dir(k, 0, ix, iy, iz, phi, lns), gy = diff_s_dir(k, 1, ix, iy, iz, phi, lns), gz = diff_s_dir(k, 2, ix, iy, iz, phi, lns);
+      if (lns) { b[idx].x = b[idx].x - gx; b[idx].y = b[idx].y - gy; b[idx].z = b[idx].z - gz; }
+      else { b[idx].x = b[idx].x + gx; b[idx].y = b[idx].y + gy; b[idx].z = b[idx].z + gz; }
+    END_RANGE
+    return 0;
+  }
+  if (!strcmp(name, "calc_axpy") || !strcmp(name, "calc_xpay")) { /* kernel_CG.cl:8-30 */
+    if (nargs < 3) return -2;
+    REAL *x = (REAL *)args[0];
+    const REAL *y = (REAL *)args[1];
+    REAL a = ((REAL *)args[2])[0];
+    int axpy = !strcmp(name, "calc_axpy");
+    FOR_RANGE(B)
+      x[idx] = axpy ? a * x[idx] + y[idx] : x[idx] + a * y[idx];
+    END_RANGE
+    return 0;
+  }
+  return -1;
+}
+
+/* ---- exported entry points ---- */
+
+/* Run one reference kernel by name over the NDRange g (local size l for the reductions).
+ * Returns 0 on success, -1 unknown kernel, -2 wrong argument count, -3 bad range. */
+int ieqo_run_kernel(const ieqo_cfg *cfg, const char *name, const uint32_t g[3], const uint32_t l[3], void **args, int nargs) {
+  ctx_t k;
+  k.c = cfg;
+  k.op = &ieq_opsets[cfg->opset];
+  k.nx = (uint32_t)cfg->nx;
+  k.ny = (uint32_t)cfg->ny;
+  k.nz = (uint32_t)cfg->nz;
+  set_threads(cfg);
+  for (int i = 0; i < NSTAGES; i++)
+    if (!strcmp(STAGES[i].name, name)) {
+      if (nargs != STAGES[i].nf + 4) return -2;
+      return run_stage(&k, &STAGES[i], g, args);
+    }
+  return run_other(&k, name, g, l, args, nargs);
+}
+
+/* Run a list of kernels on one argument list, like a MatCL kernel-list call. */
+int ieqo_run_list(const ieqo_cfg *cfg, const char *const *names, int n, const uint32_t g[3], const uint32_t l[3], void **args,
+                  int nargs) {
+  for (int i = 0; i < n; i++) {
+    int rc = ieqo_run_kernel(cfg, names[i], g, l, args, nargs);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+/* The right-hand side DB at every node (diagnostic; not a reference kernel). */
+int ieqo_rhs(const ieqo_cfg *cfg, const void *b, const void *J, const void *u, double t, void *out) {
+  ctx_t k;
+  k.c = cfg;
+  k.op = &ieq_opsets[cfg->opset];
+  k.nx = (uint32_t)cfg->nx;
+  k.ny = (uint32_t)cfg->ny;
+  k.nz = (uint32_t)cfg->nz;
+  set_threads(cfg);
+  uint32_t g[3] = {k.nx, k.ny, k.nz};
+  const ctx_t *kp = &k;
+  {
+    const ctx_t *k = kp;
+    FOR_RANGE(0)
+      ((real4 *)out)[idx] = DB(k, ix, iy, iz, (const real4 *)b, (const real4 *)J, (const real4 *)u, (REAL)t);
+    END_RANGE
+  }
+  return 0;
+}
+
+int ieqo_sizeof_real(void) { return (int)sizeof(REAL); }
+int ieqo_sizeof_cfg(void) { return (int)sizeof(ieqo_cfg); }
+int ieqo_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

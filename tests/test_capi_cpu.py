@@ -1,0 +1,121 @@
+"""The C-ABI library without a GPU: exports, settings parsing, error behaviour.
+
+Nothing here launches a kernel; program build and kernel-name validation are host
+code.  Error messages mirror the reference's compile-time #error diagnostics
+(induction_Eq_volume.cl:22,35,48,68,81,94,114,127,140; kernel_div_cleaning.cl:52;
+*OrderExtended.h NS_D0 #error).
+"""
+import ctypes
+import os
+import re
+
+import pytest
+
+from inductioneq_amd import _capi
+from inductioneq_amd.settings import ProgramConfig, generate_settings
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_capi.LIB_PATH):
+        from inductioneq_amd import build
+
+        build.build(verbose=False)
+    return _capi.load()
+
+
+def test_every_declared_symbol_is_exported(lib):
+    hdr = open(os.path.join(ROOT, "include", "inductioneq.h")).read()
+    declared = set(re.findall(r"\b(ieq_\w+)\s*\(", hdr))
+    assert declared == set(_capi.EXPORTS)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def _build(lib, settings, modules):
+    h = ctypes.c_void_p()
+    mods = (ctypes.c_char_p * len(modules))(*[m.encode() for m in modules])
+    rc = lib.ieq_program_build(ctypes.byref(h), settings.encode(), mods, len(modules))
+    return rc, h
+
+
+def test_build_and_info_roundtrip(lib):
+    cfg = ProgramConfig(real="double", opset="6thOrder", nx=40, ny=36, nz=12, periodic=True, hall=True,
+                        dissipation="HIGH_ORDER", ho_factor=0.5, testcase="hall_periodic", w_size=16,
+                        xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1, dt=1e-3)
+    rc, h = _build(lib, cfg.settings_string(), cfg.modules())
+    assert rc == 0, lib.ieq_last_error(None)
+    info = _capi.ProgramInfo()
+    assert lib.ieq_program_get_info(h, ctypes.byref(info)) == 0
+    assert (info.nx, info.ny, info.nz) == (40, 36, 12)
+    assert info.real_bytes == 8 and info.opset == 2 and info.order == 6 and info.num_bounds == 6
+    assert info.periodic == 1 and info.hall == 1 and info.dissipation == 1 and info.ho_factor == 0.5
+    assert info.testcase == 2 and info.w_size == 16
+    assert info.dx == cfg.dx and info.dt == 1e-3
+    assert info.num_nodes == 40 * 36 * 12 and info.num_nodes_pad == cfg.num_nodes_pad
+    assert info.stride % 64 == 0 and info.stride >= info.num_nodes_pad
+    assert lib.ieq_vector_field_bytes(h) == 3 * info.stride * 8
+    lib.ieq_program_release(h)
+
+
+def test_settings_string_roundtrip():
+    cfg = ProgramConfig(real="float", opset="4thOrderExtended", nx=20, ny=20, nz=10, hall=False,
+                        dissipation="ADAPTIVE", form_uibj="SPLIT", form_source="ZERO", form_ujbi="PRODUCT",
+                        testcase="rotation_3D", laplace="WIDE_STENCIL_LNS", dt=0.01)
+    back = ProgramConfig.from_settings(cfg.settings_string(), cfg.modules())
+    for f in ("real", "opset", "nx", "ny", "nz", "dx", "dy", "dz", "dt", "periodic", "hall", "dissipation", "form_uibj",
+              "form_source", "form_ujbi", "testcase", "laplace", "bnodes"):
+        assert getattr(back, f) == getattr(cfg, f), f
+
+
+def test_generate_settings_formats():
+    # matlab/generate_settings.m: USE strings, %d integers, %.16e floats, optimizations verbatim
+    s = generate_settings({"a": "USE_HALL", "N": 40, "DX": 0.05, "optimizations": " -cl-mad-enable", "REAL": "double"},
+                          ["a", "N", "DX", "optimizations", "REAL"])
+    assert s == " -DUSE_HALL=1 -DN=40 -DDX=5.0000000000000003e-02 -cl-mad-enable -DREAL=double"
+
+
+@pytest.mark.parametrize("drop,msg", [
+    ("-DUSE_UIBJ_PRODUCT=1", "No discretization of 'D_j(u_i B_j)' specified"),
+    ("-DUSE_SOURCE_CENTRAL=1", "No discretization of 'u_i D_j B_j' specified"),
+    ("-DUSE_UJBI_CENTRAL=1", "No discretization of 'D_j(u_j B_i)' specified"),
+    ("-DREAL=double", "REAL must be defined"),
+])
+def test_missing_defines_fail_like_the_reference(lib, drop, msg):
+    cfg = ProgramConfig(testcase="rotation_2D")
+    rc, _ = _build(lib, cfg.settings_string().replace(" " + drop, ""), cfg.modules())
+    assert rc == _capi.IEQ_ERR_SETTINGS
+    assert msg in lib.ieq_last_error(None).decode()
+
+
+def test_narrow_dirichlet_laplacian_rejected_for_extended(lib):
+    cfg = ProgramConfig(opset="4thOrderExtended", laplace="NARROW_STENCIL_DIRICHLET", testcase="rotation_2D")
+    rc, _ = _build(lib, cfg.settings_string(), cfg.modules())
+    assert rc == _capi.IEQ_ERR_SETTINGS
+    assert "narrow stencil operator" in lib.ieq_last_error(None).decode()
+
+
+def test_divergence_cleaning_needs_a_laplacian(lib):
+    cfg = ProgramConfig(testcase="rotation_2D", laplace="WIDE_STENCIL_DIRICHLET")
+    s = cfg.settings_string().replace(" -DUSE_LAPLACE_WIDE_STENCIL_DIRICHLET=1", "")
+    rc, _ = _build(lib, s, cfg.modules())
+    assert rc == _capi.IEQ_ERR_SETTINGS
+    assert "Define a discretization of the Laplace operator" in lib.ieq_last_error(None).decode()
+
+
+def test_unknown_and_missing_kernels(lib):
+    cfg = ProgramConfig(testcase="rotation_2D")  # no divergence-cleaning module
+    rc, h = _build(lib, cfg.settings_string(), cfg.modules())
+    assert rc == 0
+    g = (ctypes.c_uint32 * 3)(4, 4, 4)
+    args = (ctypes.c_void_p * 2)(1, 2)
+    for name in (b"no_such_kernel", b"calc_laplace_divcleaning"):
+        names = (ctypes.c_char_p * 1)(name)
+        rc = lib.ieq_run(h, names, 1, g, g, args, 2, None)
+        assert rc == _capi.IEQ_ERR_KERNEL
+        assert "kernel not in program" in lib.ieq_last_error(h).decode()
+    names = (ctypes.c_char_p * 1)(b"SSPRK33_1")
+    assert lib.ieq_run(h, names, 1, g, g, args, 2, None) == _capi.IEQ_ERR_ARGS  # needs 7 arguments
+    lib.ieq_program_release(h)

@@ -1,0 +1,221 @@
+"""HIP library vs the CPU oracle, kernel by kernel, through the C ABI.
+
+Bar: FP64 results agree with the oracle to 1e-12 relative (max norm over the field,
+per kernel / per RK step); FP32 to 2e-5.  Inputs are seeded smooth-plus-noise fields
+on grids the oracle finishes in milliseconds.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from inductioneq_amd.schedules import INTEGRATORS, rk_schedule
+from inductioneq_amd.settings import ProgramConfig
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"double": 1e-12, "float": 2e-5}
+
+
+def _pair(cfg):
+    from pyoracle import Oracle
+
+    from inductioneq_amd.program import Program
+
+    return Program(cfg), Oracle(cfg)
+
+
+def _fields(cfg, seed, smooth=True):
+    rng = np.random.default_rng(seed)
+    n = cfg.num_nodes
+    x = np.arange(cfg.nx) * cfg.dx + cfg.xmin
+    y = np.arange(cfg.ny) * cfg.dy + cfg.ymin
+    z = np.arange(cfg.nz) * cfg.dz + cfg.zmin
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    base = np.stack([np.sin(X + 0.3) * np.cos(Y), np.cos(Z - 0.2) * np.sin(X), np.sin(Y + Z)], -1).reshape(-1, 3)
+    B = (base if smooth else 0) + 0.05 * rng.standard_normal((n, 3))
+    U = np.stack([np.cos(Y), np.sin(Z) + 0.5, np.cos(X) * 0.3], -1).reshape(-1, 3) + 0.05 * rng.standard_normal((n, 3))
+    rho = 1.0 + 0.2 * rng.random(n)
+    return B, U, rho
+
+
+def _aos(cfg, a3):
+    out = np.zeros((cfg.num_nodes_pad, 4), cfg.dtype)
+    out[: a3.shape[0], :3] = a3
+    return out
+
+
+def _scal(cfg, a):
+    out = np.zeros((cfg.num_nodes_pad,), cfg.dtype)
+    out[: a.shape[0]] = a
+    return out
+
+
+def _relerr(got, want):
+    scale = max(np.abs(want).max(), 1e-300)
+    return np.abs(got - want).max() / scale
+
+
+def _cmp_vec(prog, dev, host_aos, cfg, tol, what):
+    got = prog.download_real4(dev)[: cfg.num_nodes, :3].astype(np.float64)
+    want = host_aos[: cfg.num_nodes, :3].astype(np.float64)
+    e = _relerr(got, want)
+    assert e <= tol, "%s: rel err %.3e > %.1e" % (what, e, tol)
+    return e
+
+
+CASES = [
+    # (opset, n, periodic, hall, dissipation, forms, testcase, real)
+    ("4thOrder", 14, False, False, None, ("PRODUCT", "CENTRAL", "CENTRAL"), "rotation_2D", "double"),
+    ("6thOrder", 14, True, True, "HIGH_ORDER", ("PRODUCT", "CENTRAL", "CENTRAL"), "hall_periodic", "double"),
+    ("2ndOrder", 12, False, True, "FIRST_ORDER", ("SPLIT", "SPLIT", "SPLIT"), "confined_domain", "double"),
+    ("6thOrderExtended", 18, False, False, "HIGH_ORDER", ("CENTRAL", "ZERO", "PRODUCT"), "rotation_3D", "double"),
+    ("4thOrderExtended", 14, False, True, "ADAPTIVE", ("PRODUCT", "SPLIT", "CENTRAL"), "hall_travelling_wave", "double"),
+    ("2ndOrderExtended", 12, True, False, None, ("CENTRAL", "CENTRAL", "SPLIT"), "alfven_periodic_2D", "double"),
+    ("6thOrder", 14, True, True, "HIGH_ORDER", ("PRODUCT", "CENTRAL", "CENTRAL"), "hall_periodic", "float"),
+    ("4thOrder", 14, False, False, None, ("PRODUCT", "CENTRAL", "CENTRAL"), "gaussian_rotation_3D", "float"),
+]
+
+
+def _cfg(case, dt=None):
+    ops, n, per, hall, diss, forms, tc, real = case
+    lo, hi = (0.0, 2 * math.pi) if per else (-1.0, 1.0)
+    cfg = ProgramConfig(real=real, opset=ops, nx=n, ny=n + 1, nz=n - 1, xmin=lo, xmax=hi, ymin=lo, ymax=hi, zmin=lo, zmax=hi,
+                        periodic=per, hall=hall, dissipation=diss, form_uibj=forms[0], form_source=forms[1],
+                        form_ujbi=forms[2], testcase=tc, w_size=64, laplace="WIDE_STENCIL_DIRICHLET", mp2min=1e-4,
+                        mp2max=1e-2, cmin=1.0, cmax=10.0, ho_factor=0.7)
+    cfg.dt = dt if dt is not None else 0.05 * min(cfg.dx, cfg.dy, cfg.dz) ** 2
+    return cfg
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s-%s" % (c[0], c[6], c[7]))
+@pytest.mark.parametrize("integrator", INTEGRATORS)
+def test_one_step_all_integrators(case, integrator):
+    cfg = _cfg(case)
+    prog, orc = _pair(cfg)
+    variable_u = case[6] in ("hall_periodic", "hall_travelling_wave", "alfven_periodic_2D")
+    names, nf = rk_schedule(integrator, cfg.hall, variable_u)
+    B, U, rho = _fields(cfg, seed=7)
+    hb = [_aos(cfg, B)] + [orc.vec() for _ in range(nf - 1)]
+    hJ, hu, hr, ht = orc.vec(), _aos(cfg, U), _scal(cfg, rho), orc.time(0.25)
+    db = [prog.upload_real4(h) for h in hb]
+    dJ, du, dr = prog.upload_real4(hJ), prog.upload_real4(hu), prog.upload_real(hr)
+    import torch
+
+    dt_ = torch.tensor([0.25, 0.25], dtype=prog.torch_dtype, device="cuda")
+    orc.run(names * 2, hb + [hJ, hu, hr, ht])
+    prog.run(names * 2, db + [dJ, du, dr, dt_])
+    torch.cuda.synchronize()
+    tol = TOL[cfg.real] * (20 if integrator == "SSPRK104" else 1)
+    for i in range(nf):
+        _cmp_vec(prog, db[i], hb[i], cfg, tol, "register %d" % i)
+    assert np.allclose(dt_.cpu().numpy()[:2], ht[:2], rtol=1e-12 if cfg.real == "double" else 1e-6, atol=0)
+
+
+@pytest.mark.parametrize("case", CASES[:6], ids=lambda c: "%s-%s" % (c[0], c[6]))
+def test_operator_kernels(case):
+    cfg = _cfg(case)
+    prog, orc = _pair(cfg)
+    B, U, rho = _fields(cfg, seed=3)
+    tol = TOL[cfg.real]
+    hb = _aos(cfg, B)
+    db = prog.upload_real4(hb)
+    # calc_div / calc_curl / calc_grad (kernel_operator.cl)
+    hd, dd = orc.scal(), prog.scalar()
+    orc.run("calc_div", [hb, hd])
+    prog.run("calc_div", [db, dd])
+    assert _relerr(prog.download_real(dd)[: cfg.num_nodes], hd[: cfg.num_nodes]) <= tol
+    hc, dc = orc.vec(), prog.vector()
+    orc.run("calc_curl", [hb, hc])
+    prog.run("calc_curl", [db, dc])
+    _cmp_vec(prog, dc, hc, cfg, tol, "curl")
+    s = _scal(cfg, rho)
+    hg, dg = orc.vec(), prog.vector()
+    orc.run("calc_grad", [s, hg])
+    prog.run("calc_grad", [prog.upload_real(s), dg])
+    _cmp_vec(prog, dg, hg, cfg, tol, "grad")
+    # curl B / rho
+    hJ, dJ = orc.vec(), prog.vector()
+    args_h = [hb, orc.vec(), orc.vec(), hJ, _aos(cfg, U), s, orc.time()]
+    import torch
+
+    args_d = [db, prog.vector(), prog.vector(), dJ, prog.upload_real4(_aos(cfg, U)), prog.upload_real(s), prog.time_buffer()]
+    orc.run("calc_curlB_rho_1_3args", args_h)
+    prog.run("calc_curlB_rho_1_3args", args_d)
+    torch.cuda.synchronize()
+    _cmp_vec(prog, dJ, hJ, cfg, tol, "curlB_rho")
+    # reductions (per-group partials, W_SIZE = 64)
+    for name, fields_h, fields_d in [("norm2", [hb], [db]), ("norm2_diff", [hb, hJ], [db, dJ]), ("norm2_S", [s], [prog.upload_real(s)]),
+                                     ("dot_product", [s, s], [prog.upload_real(s), prog.upload_real(s)])]:
+        ho = np.zeros((cfg.num_groups,), cfg.dtype)
+        orc.run(name, fields_h + [ho], g=(cfg.num_nodes_pad, 1, 1), l=(cfg.w_size, 1, 1))
+        do = prog.reduce_output()
+        prog.run(name, fields_d + [do], g=(cfg.num_nodes_pad, 1, 1), l=(cfg.w_size, 1, 1))
+        got = do.cpu().numpy()
+        assert _relerr(got, ho) <= 10 * tol, name
+
+
+@pytest.mark.parametrize("ops", ["2ndOrder", "4thOrder", "6thOrder", "2ndOrderExtended", "4thOrderExtended", "6thOrderExtended"])
+@pytest.mark.parametrize("lap", ["WIDE_STENCIL_DIRICHLET", "NARROW_STENCIL_DIRICHLET", "WIDE_STENCIL_LNS"])
+def test_divergence_cleaning_kernels(ops, lap):
+    if lap == "NARROW_STENCIL_DIRICHLET" and "Extended" in ops:
+        pytest.skip("NS_D0 is not defined for extended operators (reference #error)")
+    n = 24
+    cfg = ProgramConfig(opset=ops, nx=n, ny=n - 2, nz=n + 1, laplace=lap, w_size=64, testcase="rotation_2D")
+    prog, orc = _pair(cfg)
+    B, U, rho = _fields(cfg, seed=11)
+    hb = _aos(cfg, B)
+    db = prog.upload_real4(hb)
+    bn = cfg.bnodes
+    g_dc = (cfg.nx - 2 * bn, cfg.ny - 2 * bn, cfg.nz - 2 * bn)
+    hd, dd = orc.scal(), prog.scalar()
+    orc.run("calc_div_divcleaning", [hb, hd], g=g_dc)
+    prog.run("calc_div_divcleaning", [db, dd], g=g_dc)
+    assert _relerr(prog.download_real(dd), hd) <= 1e-12
+    s = _scal(cfg, rho - 1.0)
+    hl, dl = orc.scal(), prog.scalar()
+    orc.run("calc_laplace_divcleaning", [s, hl], g=g_dc)
+    prog.run("calc_laplace_divcleaning", [prog.upload_real(s), dl], g=g_dc)
+    assert _relerr(prog.download_real(dl), hl) <= 1e-12
+    orc.run("projector_divcleaning", [hb, s], g=(cfg.nx, cfg.ny, cfg.nz))
+    prog.run("projector_divcleaning", [db, prog.upload_real(s)], g=(cfg.nx, cfg.ny, cfg.nz))
+    _cmp_vec(prog, db, hb, cfg, 1e-12, "projector")
+    import torch
+
+    for name in ("calc_axpy", "calc_xpay"):
+        hx, hy = s.copy(), _scal(cfg, rho)
+        dx_, dy_ = prog.upload_real(hx), prog.upload_real(hy)
+        a_h = np.array([-0.37], cfg.dtype)
+        orc.run(name, [hx, hy, a_h], g=g_dc)
+        prog.run(name, [dx_, dy_, torch.tensor([-0.37], dtype=prog.torch_dtype, device="cuda")], g=g_dc)
+        assert _relerr(prog.download_real(dx_), hx) <= 1e-15
+
+
+@pytest.mark.parametrize("ops", ["2ndOrder", "4thOrder", "6thOrder", "6thOrderExtended"])
+@pytest.mark.parametrize("integrator", INTEGRATORS)
+@pytest.mark.parametrize("hall,diss,u", [(True, "HIGH_ORDER", (0.0, 0.0, 0.0)), (True, None, (0.3, -0.2, 0.1)),
+                                         (False, "HIGH_ORDER", (0.5, 0.2, -0.4))])
+def test_fused_step_matches_reference_schedule(ops, integrator, hall, diss, u):
+    """ieq_step on a periodic grid with uniform u and rho (fused kernels) vs the oracle's schedule."""
+    n = 20
+    cfg = ProgramConfig(opset=ops, nx=n, ny=n + 4, nz=n - 2, xmin=0, xmax=2 * math.pi, ymin=0, ymax=2 * math.pi, zmin=0,
+                        zmax=2 * math.pi, periodic=True, hall=hall, dissipation=diss, testcase="uniform_flow", ux=u[0],
+                        uy=u[1], uz=u[2], ho_factor=0.5, w_size=64)
+    cfg.dt = 0.02 * min(cfg.dx, cfg.dy, cfg.dz) ** 2
+    prog, orc = _pair(cfg)
+    names, nf = rk_schedule(integrator, hall, False)
+    B, _, _ = _fields(cfg, seed=5)
+    hb = [_aos(cfg, B)] + [orc.vec() for _ in range(nf - 1)]
+    U = np.tile(np.array(u), (cfg.num_nodes, 1))
+    rho = np.full(cfg.num_nodes, 1.25)
+    hJ, hu, hr, ht = orc.vec(), _aos(cfg, U), _scal(cfg, rho), orc.time(0.0)
+    nsteps = 3
+    orc.run(names * nsteps, hb + [hJ, hu, hr, ht])
+    db = [prog.upload_real4(h) for h in [_aos(cfg, B)] + [orc.vec() for _ in range(nf - 1)]]
+    import torch
+
+    dt_ = prog.time_buffer(0.0)
+    prog.step(integrator, nsteps, db + [prog.vector(), prog.upload_real4(hu), prog.upload_real(hr), dt_])
+    torch.cuda.synchronize()
+    _cmp_vec(prog, db[0], hb[0], cfg, 1e-12 * (20 if integrator == "SSPRK104" else 1), "b1 after %d steps" % nsteps)
+    assert abs(float(dt_[0]) - float(ht[0])) <= 1e-15 and abs(float(dt_[1]) - float(ht[1])) <= 1e-15

@@ -1,0 +1,112 @@
+"""The coefficient tables against what the mathematics fixes.
+
+SBP property Q + Q^T = diag(-1, 0, ..., 0, 1) with Q = M D; polynomial exactness of D
+(degree p in the interior, p/2 at the boundary rows); the tabulated adjoint equals
+M^-1 D^T M; the -Laplacians equal D (M^-1 D^T M) (least norm) and -D D on the interior
+(Dirichlet); the narrow Dirichlet Laplacian is M-symmetric positive definite; the
+high-order dissipation is M-symmetric with interior stencil (-1)^(p+1) delta^(2p).
+A wrong sign, a transposed row or a dropped entry in the tables fails one of these.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from sbp_matrices import ORDER, OPSETS, assemble, hod_matrix, load_tables, norm_matrix
+
+T = load_tables()
+N = 40
+
+
+@pytest.mark.parametrize("ops", OPSETS)
+def test_sbp_property(ops):
+    t = T[ops]
+    D = assemble(t["diff"], N)
+    M = norm_matrix(t["minv"], N)
+    Q = M @ D
+    Bd = np.zeros((N, N))
+    Bd[0, 0], Bd[-1, -1] = -1.0, 1.0
+    assert np.abs(Q + Q.T - Bd).max() < 1e-14
+
+
+@pytest.mark.parametrize("ops", OPSETS)
+def test_polynomial_exactness(ops):
+    t = T[ops]
+    p = ORDER[ops]
+    nb = (t["diff"].shape[0] - 1) // 2
+    x = np.linspace(-1.0, 1.0, N)
+    h = x[1] - x[0]
+    D = assemble(t["diff"], N) / h
+    for k in range(0, p + 1):
+        err = D @ x**k - (k * x ** (k - 1) if k else 0 * x)
+        interior = np.abs(err[nb:-nb]).max()
+        assert interior < 1e-9, (k, interior)
+        if k <= p // 2:
+            assert np.abs(err).max() < 1e-9, (k, np.abs(err).max())
+    # and not exact one degree higher in the interior (order is exactly p)
+    err = D @ x ** (p + 1) - (p + 1) * x**p
+    assert np.abs(err[nb:-nb]).max() > 1e-8
+
+
+@pytest.mark.parametrize("ops", OPSETS)
+def test_adjoint_table(ops):
+    t = T[ops]
+    D = assemble(t["diff"], N)
+    Dadj = assemble(t["diff_adj"], N)
+    M = norm_matrix(t["minv"], N)
+    assert np.abs(Dadj - np.linalg.inv(M) @ D.T @ M).max() < 1e-14
+
+
+@pytest.mark.parametrize("ops", OPSETS)
+def test_laplacians(ops):
+    t = T[ops]
+    D = assemble(t["diff"], N)
+    Dadj = assemble(t["diff_adj"], N)
+    L_ln = assemble(t["lap_ws_ln"], N)
+    assert np.abs(L_ln - D @ Dadj).max() < 1e-13
+    L_d0 = assemble(t["lap_ws_d0"], N)
+    assert np.abs(L_d0[0]).max() == 0 and np.abs(L_d0[-1]).max() == 0
+    assert np.abs((L_d0 + D @ D)[1:-1, 1:-1]).max() < 1e-13
+    if "lap_ns_d0" in t:
+        L = assemble(t["lap_ns_d0"], N)[1:-1, 1:-1]
+        M = norm_matrix(t["minv"], N)[1:-1, 1:-1]
+        ML = M @ L
+        assert np.abs(ML - ML.T).max() < 1e-13
+        assert np.linalg.eigvalsh(0.5 * (ML + ML.T)).min() > 0
+
+
+@pytest.mark.parametrize("ops", OPSETS)
+def test_high_order_dissipation(ops):
+    t = T[ops]
+    p = ORDER[ops] // 2
+    A = hod_matrix(t, N, ORDER[ops])
+    M = norm_matrix(t["minv"], N)
+    MA = M @ A
+    assert np.abs(MA - MA.T).max() < 1e-12
+    # interior stencil (-1)^(p+1) delta^(2p)
+    delta = np.array([1.0])
+    for _ in range(2 * p):  # coefficients of (1 - shift)^(2p)
+        delta = np.convolve(delta, [1.0, -1.0])
+    want = (-1.0) ** (p + 1) * delta
+    i = N // 2
+    assert np.abs(A[i, i - p : i + p + 1] - want).max() < 1e-14
+    # periodic: negative semidefinite
+    Ap = hod_matrix(t, 32, ORDER[ops], periodic=True)
+    assert np.linalg.eigvalsh(0.5 * (Ap + Ap.T)).max() < 1e-12
+    if ORDER[ops] < 6:  # bounded grid: dissipative for 2nd and 4th order
+        assert np.linalg.eigvalsh(0.5 * (MA + MA.T)).max() < 1e-12
+
+
+def test_tables_match_reference_headers(tmp_path):
+    """Regenerating from the reference headers reproduces the committed header."""
+    ref = "/root/reference"
+    if not os.path.isdir(os.path.join(ref, "include")):
+        pytest.skip("reference tree not mounted")
+    out = tmp_path / "t.h"
+    gen = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "inductioneq_amd", "csrc", "gen_tables.py")
+    subprocess.run([sys.executable, gen, ref, str(out)], check=True, capture_output=True)
+    from sbp_matrices import TABLES_H
+
+    assert open(out).read() == open(TABLES_H).read()
