@@ -34,13 +34,31 @@ def parse():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--integrator", default="CarpenterKennedy2N54")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="RK steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=15, help="RK steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB mebibytes (calibrates the HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE give the counter scale."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
 
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the workload is single-GPU; multi-GPU domain "
               "decomposition is design-only); run with --gpus 1", file=sys.stderr)

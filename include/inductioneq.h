@@ -125,6 +125,64 @@ int ieq_run(ieq_program *prog, const char *const *names, int n, const uint32_t g
 int ieq_step(ieq_program *prog, const char *integrator, long long nsteps, int variable_u, void *const *args, int nargs,
              double *kernel_seconds);
 
+/* Result of ieq_clean_div_cg. */
+typedef struct {
+  double divergence_norm_initial; /* sqrt(r.r), r = div B on the BNODES interior (clean_div_CG.m:36-44) */
+  double divergence_norm_final;   /* SBP-norm of div B after the projection (clean_div_CG.m:164-167) */
+  double residual_norm;           /* sqrt(r.r) when the loop stopped */
+  int iterations;
+  double seconds;                 /* device time of the CG loop */
+} ieq_cg_result;
+
+/* Divergence cleaning of the vector field b (in place) with the program's -Laplacian
+ * (USE_LAPLACE_*, BNODES).  Replaces the host-driven loop of
+ * matlab/+divergence_cleaning/clean_div_CG.m:31-86 (and +induction_eq/clean_div.m):
+ * conjugate gradients on -Lap phi = div B with the same update sequence (u = beta*u + r,
+ * alpha = r.r / u.q, beta = r.r / old, old = 1 initially), stopping when
+ * r.r <= abs_threshold^2 (the reference's test) or, when rel_tolerance > 0, when
+ * |r| <= rel_tolerance * |r0|, or after max_iterations; then B += grad phi
+ * (B -= grad_adj phi for WS_LN) over the whole grid.  alpha and beta stay on the
+ * device and the dot products accumulate in FP64 for both REAL types.  phi receives
+ * the potential (scalar field). */
+int ieq_clean_div_cg(ieq_program *prog, void *b, void *phi, double abs_threshold, double rel_tolerance, int max_iterations,
+                     ieq_cg_result *result);
+
+/* One member of an ensemble: an independent periodic solve with its own grid, spacing,
+ * time step, uniform density (Hall coefficient 1/rho) and uniform velocity.  Fields are
+ * device vector fields in this library's layout with component stride
+ * ieq_member_stride(nx, ny, nz).  b3 is the third register of three-register schemes
+ * and the ping-pong register of the 2N schemes; emf is a work register. */
+typedef struct {
+  int nx, ny, nz;
+  double dx, dy, dz, dt;
+  double rho;
+  double ux, uy, uz;
+  void *b1, *b2, *b3, *emf;
+} ieq_member;
+
+typedef struct ieq_ensemble ieq_ensemble;
+
+/* Component stride (elements) of a member field: nx*ny*nz rounded up to 64. */
+long long ieq_member_stride(int nx, int ny, int nz);
+
+/* Create an ensemble of independent periodic solves sharing the program's settings
+ * (REAL, operator order, USE_HALL, dissipation, flux forms; the program must be
+ * periodic) and one integrator (SSPRK33, CarpenterKennedy2N54 or ToulorgeDesmet2N84F).
+ * No counterpart in the reference (one program per solve there).  Each RK stage of the
+ * whole ensemble is one persistent launch per fused kernel over a tile list covering
+ * every member; on the first ieq_ensemble_step the tile shape, z-chunk and tile order are
+ * chosen by timing the candidates (member fields are saved and restored around it). */
+int ieq_ensemble_create(ieq_ensemble **ens, ieq_program *prog, const ieq_member *members, int nmembers, const char *integrator);
+
+/* Advance every member by nsteps steps of its own dt (hipGraph replay of one step, or two
+ * for the 2N schemes).  b1 of every member holds the state on return. */
+int ieq_ensemble_step(ieq_ensemble *ens, long long nsteps, double *kernel_seconds);
+
+/* Chosen launch configuration: tile rows, z-chunk, order (0 member order, 1 largest first). */
+int ieq_ensemble_config(const ieq_ensemble *ens, int *tile_rows, int *z_chunk, int *order);
+
+void ieq_ensemble_release(ieq_ensemble *ens);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,7 +8,7 @@ import ctypes
 import os
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG, "libinductioneq_hip.so")
+LIB_PATH = os.environ.get("IEQ_LIB_PATH") or os.path.join(_PKG, "libinductioneq_hip.so")
 
 IEQ_OK = 0
 IEQ_ERR_SETTINGS = -1
@@ -22,7 +22,24 @@ EXPORTS = [
     "ieq_program_build", "ieq_program_release", "ieq_last_error", "ieq_program_get_info", "ieq_set_stream",
     "ieq_synchronize", "ieq_alloc", "ieq_free", "ieq_vector_field_bytes", "ieq_scalar_field_bytes",
     "ieq_upload_real4", "ieq_download_real4", "ieq_upload_real", "ieq_download_real", "ieq_run", "ieq_step",
+    "ieq_clean_div_cg", "ieq_member_stride", "ieq_ensemble_create", "ieq_ensemble_step", "ieq_ensemble_config",
+    "ieq_ensemble_release",
 ]
+
+_NOT_INT = ("ieq_program_release", "ieq_last_error", "ieq_vector_field_bytes", "ieq_scalar_field_bytes",
+            "ieq_member_stride", "ieq_ensemble_release")
+
+
+class CgResult(ctypes.Structure):
+    _fields_ = [("divergence_norm_initial", ctypes.c_double), ("divergence_norm_final", ctypes.c_double),
+                ("residual_norm", ctypes.c_double), ("iterations", ctypes.c_int), ("seconds", ctypes.c_double)]
+
+
+class Member(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double), ("rho", ctypes.c_double),
+                ("ux", ctypes.c_double), ("uy", ctypes.c_double), ("uz", ctypes.c_double), ("b1", ctypes.c_void_p),
+                ("b2", ctypes.c_void_p), ("b3", ctypes.c_void_p), ("emf", ctypes.c_void_p)]
 
 
 class ProgramInfo(ctypes.Structure):
@@ -76,8 +93,17 @@ def load() -> ctypes.CDLL:
                             ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     lib.ieq_step.argtypes = [P, ctypes.c_char_p, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(P), ctypes.c_int,
                              ctypes.POINTER(ctypes.c_double)]
+    lib.ieq_clean_div_cg.argtypes = [P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(CgResult)]
+    lib.ieq_member_stride.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.ieq_member_stride.restype = ctypes.c_longlong
+    lib.ieq_ensemble_create.argtypes = [ctypes.POINTER(P), P, ctypes.POINTER(Member), ctypes.c_int, ctypes.c_char_p]
+    lib.ieq_ensemble_step.argtypes = [P, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double)]
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.ieq_ensemble_config.argtypes = [P, ip, ip, ip]
+    lib.ieq_ensemble_release.argtypes = [P]
+    lib.ieq_ensemble_release.restype = None
     for fn in EXPORTS:
-        if fn not in ("ieq_program_release", "ieq_last_error", "ieq_vector_field_bytes", "ieq_scalar_field_bytes"):
+        if fn not in _NOT_INT:
             getattr(lib, fn).restype = ctypes.c_int
     _lib = lib
     return lib

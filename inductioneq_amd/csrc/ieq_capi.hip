@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/inductioneq.h"
+#include "ieq_cg.hpp"
 #include "ieq_device.hpp"
 #include "ieq_fused.hpp"
 #include "ieq_generic.hpp"
@@ -166,6 +167,7 @@ struct ieq_program {
   hipStream_t stream = nullptr;
   std::string err;
   ieq::FusedWork* fused = nullptr;
+  ieq::CgWork* cg = nullptr;
 
   bool has(const char* m) const { return modules.count(m) != 0; }
   int fail(int code, const std::string& msg) {
@@ -309,6 +311,17 @@ int parse_build(ieq_program* pr, const char* settings, const char* const* module
   }
   if (I.bnodes < 0 || 2 * I.bnodes >= std::min(I.nx, std::min(I.ny, I.nz)))
     return pr->fail(IEQ_ERR_SETTINGS, "BNODES out of range");
+
+  // The periodic neighbour rule wraps once (utils.h:91-98); a grid narrower than the
+  // widest stencil half-width would index outside the field.
+  if (I.periodic) {
+    int hw = (op.w - 1) / 2;
+    int nbl = 0, wl = 0;
+    if (I.laplace >= 0 && ieq::host_lap_table(I.opset, I.laplace, &nbl, &wl)) hw = std::max(hw, (wl - 1) / 2);
+    const int nmin = std::min(I.nx, std::min(I.ny, I.nz));
+    if (nmin < hw)
+      return pr->fail(IEQ_ERR_SETTINGS, "periodic grid has fewer nodes per direction than the stencil half-width");
+  }
 
   I.num_nodes = (long long)I.nx * I.ny * I.nz;
   I.num_nodes_pad = (I.num_nodes + I.w_size - 1) / I.w_size * I.w_size;
@@ -477,6 +490,7 @@ int ieq_program_build(ieq_program** prog, const char* settings, const char* cons
 void ieq_program_release(ieq_program* prog) {
   if (!prog) return;
   ieq::fused_release(prog->fused);
+  ieq::cg_release(prog->cg);
   delete prog;
 }
 
@@ -619,6 +633,152 @@ int ieq_step(ieq_program* prog, const char* integrator, long long nsteps, int va
     (void)hipEventDestroy(e1);
   }
   return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Host sum of the group partials of a reduction over the whole padded field.
+template <typename R>
+int reduce_host(ieq_program* pr, int which, const R* a, const R* b, R* partials, double* out) {
+  const ieq_program_info& I = pr->info;
+  ieq::gen_group_reduce<R>(pr->p, I.opset, pr->stream, a, b, partials, which, I.w_size, I.num_groups, I.num_nodes);
+  std::vector<R> h((size_t)I.num_groups);
+  HIP_TRY(pr, hipMemcpyAsync(h.data(), partials, h.size() * sizeof(R), hipMemcpyDeviceToHost, pr->stream));
+  HIP_TRY(pr, hipStreamSynchronize(pr->stream));
+  double acc = 0.0;
+  for (R v : h) acc += (double)v;
+  *out = acc;
+  return IEQ_OK;
+}
+
+template <typename R>
+int clean_div_t(ieq_program* pr, R* b, R* phi, double thr, double rel, int max_it, ieq_cg_result* res) {
+  const Params& p = pr->p;
+  const ieq_program_info& I = pr->info;
+  int nbl = 0, wl = 0;
+  const double* tab = ieq::host_lap_table(I.opset, I.laplace, &nbl, &wl);
+  if (!tab) return pr->fail(IEQ_ERR_SETTINGS, "no -Laplacian table for this operator set");
+  std::memset(res, 0, sizeof(*res));
+  const size_t sbytes = (size_t)I.stride * sizeof(R);
+  R* tmp = nullptr;  // scalar work field + reduction partials
+  HIP_TRY(pr, hipMalloc(&tmp, 4 * sbytes + (size_t)I.num_groups * sizeof(R) + 64));
+  R *r = tmp, *u = tmp + I.stride, *q = tmp + 2 * I.stride, *dv = tmp + 3 * I.stride, *part = tmp + 4 * I.stride;
+  int rc = ieq::cg_run_t<R>(p, I, tab, nbl, wl, pr->stream, &pr->cg, b, phi, thr, rel, max_it, 16, res, pr->err);
+  if (rc < 0) {
+    (void)hipFree(tmp);
+    return IEQ_ERR_HIP;
+  }
+  const uint32_t g_dc[3] = {(uint32_t)(p.nx - 2 * p.bnodes), (uint32_t)(p.ny - 2 * p.bnodes), (uint32_t)(p.nz - 2 * p.bnodes)};
+  if (rc == 1) {  // reference kernel sequence with host-side scalars (clean_div_CG.m:31-76)
+    HIP_TRY(pr, hipMemsetAsync(tmp, 0, 3 * sbytes, pr->stream));
+    HIP_TRY(pr, hipMemsetAsync(phi, 0, sbytes, pr->stream));
+    R* coef = part + I.num_groups;  // 1-element coefficient buffer
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, pr->stream);
+    ieq::gen_div<R>(p, I.opset, pr->stream, g_dc, b, r, p.bnodes);
+    double rr = 0, old = 1.0, uq = 0;
+    if ((rc = reduce_host<R>(pr, 3, r, r, part, &rr))) return rc;
+    const double rr0 = rr;
+    res->divergence_norm_initial = std::sqrt(rr);
+    int it = 0;
+    auto go = [&]() { return rr > thr * thr && (rel <= 0 || rr > rel * rel * rr0) && it < max_it; };
+    while (go()) {
+      R c = (R)(rr / old);
+      HIP_TRY(pr, hipMemcpyAsync(coef, &c, sizeof(R), hipMemcpyHostToDevice, pr->stream));
+      ieq::gen_axpy<R>(p, pr->stream, g_dc, u, r, coef, 0);
+      ieq::gen_lap<R>(p, I.opset, pr->stream, g_dc, u, q);
+      if ((rc = reduce_host<R>(pr, 3, u, q, part, &uq))) return rc;
+      const double alpha = rr / uq;
+      c = (R)alpha;
+      HIP_TRY(pr, hipMemcpyAsync(coef, &c, sizeof(R), hipMemcpyHostToDevice, pr->stream));
+      ieq::gen_axpy<R>(p, pr->stream, g_dc, phi, u, coef, 1);
+      c = (R)(-alpha);
+      HIP_TRY(pr, hipMemcpyAsync(coef, &c, sizeof(R), hipMemcpyHostToDevice, pr->stream));
+      ieq::gen_axpy<R>(p, pr->stream, g_dc, r, q, coef, 1);
+      old = rr;
+      if ((rc = reduce_host<R>(pr, 3, r, r, part, &rr))) return rc;
+      ++it;
+    }
+    (void)hipEventRecord(e1, pr->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    res->iterations = it;
+    res->residual_norm = std::sqrt(rr);
+    res->seconds = ms * 1e-3;
+  }
+  // projection over the whole grid and the final divergence norm (clean_div_CG.m:78-86)
+  const uint32_t g_full[3] = {(uint32_t)p.nx, (uint32_t)p.ny, (uint32_t)p.nz};
+  ieq::gen_proj<R>(p, I.opset, pr->stream, g_full, b, phi);
+  HIP_TRY(pr, hipMemsetAsync(dv, 0, sbytes, pr->stream));
+  ieq::gen_div<R>(p, I.opset, pr->stream, g_full, b, dv, 0);
+  double dn = 0;
+  if ((rc = reduce_host<R>(pr, 2, dv, nullptr, part, &dn))) return rc;
+  res->divergence_norm_final = std::sqrt(std::max(dn, 0.0));
+  HIP_TRY(pr, hipStreamSynchronize(pr->stream));
+  HIP_TRY(pr, hipFree(tmp));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(pr, e, "clean_div_cg");
+  return IEQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ieq_clean_div_cg(ieq_program* prog, void* b, void* phi, double abs_threshold, double rel_tolerance, int max_iterations,
+                     ieq_cg_result* result) {
+  if (!prog || !b || !phi || !result || max_iterations < 0) return IEQ_ERR_ARGS;
+  if (!prog->has("kernel_div_cleaning") || prog->info.laplace < 0)
+    return prog->fail(IEQ_ERR_KERNEL, "divergence cleaning needs the kernel_div_cleaning module and a USE_LAPLACE_* define");
+  if (prog->info.real_bytes == 8)
+    return clean_div_t<double>(prog, (double*)b, (double*)phi, abs_threshold, rel_tolerance, max_iterations, result);
+  return clean_div_t<float>(prog, (float*)b, (float*)phi, abs_threshold, rel_tolerance, max_iterations, result);
+}
+
+struct ieq_ensemble {
+  ieq_program* prog;
+  ieq::Ensemble* e;
+};
+
+long long ieq_member_stride(int nx, int ny, int nz) {
+  if (nx < 1 || ny < 1 || nz < 1) return 0;
+  return ((long long)nx * ny * nz + 63) / 64 * 64;
+}
+
+int ieq_ensemble_create(ieq_ensemble** ens, ieq_program* prog, const ieq_member* members, int nmembers, const char* integrator) {
+  if (!ens || !prog || !members || nmembers < 1 || !integrator) return IEQ_ERR_ARGS;
+  *ens = nullptr;
+  ieq::Ensemble* e = nullptr;
+  int rc = ieq::ensemble_create(prog->p, prog->info, members, nmembers, integrator, &e, prog->err);
+  if (rc) return rc;
+  *ens = new ieq_ensemble{prog, e};
+  return IEQ_OK;
+}
+
+int ieq_ensemble_step(ieq_ensemble* ens, long long nsteps, double* kernel_seconds) {
+  if (!ens || nsteps < 0) return IEQ_ERR_ARGS;
+  // order after the work already enqueued on the program's stream
+  HIP_TRY(ens->prog, hipStreamSynchronize(ens->prog->stream));
+  return ieq::ensemble_step(ens->e, nsteps, kernel_seconds, ens->prog->err);
+}
+
+int ieq_ensemble_config(const ieq_ensemble* ens, int* tile_rows, int* z_chunk, int* order) {
+  if (!ens || !tile_rows || !z_chunk || !order) return IEQ_ERR_ARGS;
+  ieq::ensemble_config(ens->e, tile_rows, z_chunk, order);
+  return IEQ_OK;
+}
+
+void ieq_ensemble_release(ieq_ensemble* ens) {
+  if (!ens) return;
+  ieq::ensemble_release(ens->e);
+  delete ens;
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@
 // update.  Both march along z with register queues of the z-stencil and stage one
 // xy-plane (plus halo) at a time in LDS for the x/y stencils.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "ieq_device.hpp"
@@ -135,6 +136,25 @@ template <typename R> struct FArgs {
   R cP, cQ, cD;
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8) and
+// each XCD has its own L2.  Giving XCD k the contiguous tile range
+// [start_k, start_k + count_k) keeps the halo rows a tile shares with its y- and z-neighbours
+// inside one L2 while they are being read.  Bijective for any nb.
+constexpr int kXcds = 8;
+__device__ inline int xcd_tile(int b, int nb) {
+  const int q = nb / kXcds, r = nb % kXcds, k = b % kXcds;
+  return k * q + min(k, r) + b / kXcds;
+}
+
+// Linear tile index (x fastest, then y, then z-chunk) -> tile origin.
+template <int TY> __device__ inline void tile_origin(int t, int nx, int ny, int zc, int& x0, int& y0, int& z0) {
+  const int ntx = (nx + 63) / 64, nty = (ny + TY - 1) / TY;
+  x0 = (t % ntx) * 64;
+  t /= ntx;
+  y0 = (t % nty) * TY;
+  z0 = (t / nty) * zc;
+}
+
 __device__ inline int wrap(int v, int n) {
   v %= n;
   return v < 0 ? v + n : v;
@@ -157,108 +177,195 @@ template <typename R, int RR> __device__ inline R hodsum(const StencilCoef<R>& c
   return acc;
 }
 
-// EMF kernel: E = (curl_D B / rho) x B.
+// Barrier over LDS traffic only: the LDS stores of this plane are visible after it,
+// while global prefetch loads of the next plane stay in flight across it.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Per-thread staging of one xy-plane of an NC-component field into an LDS tile with a
+// halo of RR cells.  The thread owns its column (x0+tx, y0+ty) and, at the tile edges,
+// up to one x-halo cell and up to two y-halo cells (all coordinates wrap periodically).
+template <typename R, int RR, int TY, int NC> struct PlaneStage {
+  static constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR;
+  // With TY >= 2*RR a thread stages at most one y-halo cell (top rows or bottom rows).
+  static constexpr bool ONE_Y = TY >= 2 * RR;
+  bool hx, ht, hb;
+  int lcx, lry;                // LDS column of the x-halo cell; LDS row of the y-halo cell (ONE_Y)
+  long long gx, gt, gb;        // in-plane global indices of the halo cells
+  R vx[NC], vt[NC], vb[ONE_Y ? 1 : NC];  // prefetched halo values of the next plane to stage
+
+  __device__ void init(int tx, int ty, int x0, int y0, int nx, int ny, long long NX) {
+    const int xw = wrap(x0 + tx, nx), yw = wrap(y0 + ty, ny);
+    hx = tx < RR || tx >= 64 - RR;
+    ht = ty < RR;
+    hb = ty >= TY - RR;
+    lcx = tx < RR ? tx : tx + 2 * RR;
+    gx = (tx < RR ? wrap(x0 - RR + tx, nx) : wrap(x0 + tx + RR, nx)) + NX * yw;
+    gt = xw + NX * wrap(y0 - RR + ty, ny);
+    gb = xw + NX * wrap(y0 + ty + RR, ny);
+    if (ONE_Y) {
+      lry = ht ? ty : ty + 2 * RR;
+      if (hb) gt = gb;
+      ht = ht || hb;
+    }
+  }
+  __device__ void load(const R* f, long long S, long long pz) {
+    if (hx)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) vx[c] = f[c * S + gx + pz];
+    if (ht)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) vt[c] = f[c * S + gt + pz];
+    if constexpr (!ONE_Y) {
+      if (hb)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) vb[c] = f[c * S + gb + pz];
+    }
+  }
+  __device__ void store(R (*sm)[LY][LX], int tx, int ty) {
+    if (hx)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sm[c][ty + RR][lcx] = vx[c];
+    if constexpr (ONE_Y) {
+      if (ht)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sm[c][lry][tx + RR] = vt[c];
+    } else {
+      if (ht)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sm[c][ty][tx + RR] = vt[c];
+      if (hb)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sm[c][ty + 2 * RR][tx + RR] = vb[c];
+    }
+  }
+};
+
+// EMF kernel: E = (curl_D B / rho) x B.  Software pipelined along z: plane z is
+// computed from LDS buffer (z & 1) while the loads of plane z+1 are in flight.
 template <typename R, int RR, int TY>
-__global__ __launch_bounds__(64 * TY) void k_emf(FArgs<R> a) {
-  constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR, Q = 2 * RR + 1;
-  __shared__ R sB[3][LY][LX];
+__device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int z0, R (*sB)[3][TY + 2 * RR][64 + 2 * RR]) {
+  constexpr int Q = 2 * RR + 1;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * TY, z0 = blockIdx.z * a.zc;
-  const int xg = x0 + tx, yg = y0 + ty;
-  const bool active = xg < a.nx && yg < a.ny;
-  const int xw = wrap(xg, a.nx), yw = wrap(yg, a.ny);
+  const bool active = x0 + tx < a.nx && y0 + ty < a.ny;
   const long long S = a.S, NX = a.nx, NXY = (long long)a.nx * a.ny;
+  const long long col = wrap(x0 + tx, a.nx) + NX * wrap(y0 + ty, a.ny);
   const int zend = min(z0 + a.zc, a.nz);
   const R* B = a.Y;
+  PlaneStage<R, RR, TY, 3> hs;
+  hs.init(tx, ty, x0, y0, a.nx, a.ny, NX);
 
-  R qx[Q], qy[Q];  // B_x, B_y along z at planes z-RR .. z+RR
+  R q[3][Q + 2];  // B at planes z-RR .. z+RR+2 of the own column
 #pragma unroll
-  for (int j = 0; j < Q - 1; ++j) {
-    long long id = xw + NX * yw + NXY * wrap(z0 - RR + j, a.nz);
-    qx[j] = B[id];
-    qy[j] = B[S + id];
+  for (int j = 0; j < Q + 1; ++j) {
+    const long long id = col + NXY * wrap(z0 - RR + j, a.nz);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q[c][j] = B[c * S + id];
   }
+  hs.load(B, S, NXY * z0);
   for (int z = z0; z < zend; ++z) {
-    {
-      long long id = xw + NX * yw + NXY * wrap(z + RR, a.nz);
-      qx[Q - 1] = B[id];
-      qy[Q - 1] = B[S + id];
+    const int buf = (z - z0) & 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = q[c][RR];
+    hs.store(sB[buf], tx, ty);
+    {  // prefetch plane z+RR+2 of the column and the halo of plane z+1
+      const long long id = col + NXY * wrap(z + RR + 2, a.nz);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) q[c][Q + 1] = B[c * S + id];
+      if (z + 1 < zend) hs.load(B, S, NXY * (z + 1));
     }
-    const long long pz = NXY * z;
-    const long long ic = xw + NX * yw + pz;
-    sB[0][ty + RR][tx + RR] = qx[RR];
-    sB[1][ty + RR][tx + RR] = qy[RR];
-    sB[2][ty + RR][tx + RR] = B[2 * S + ic];
-    if (tx < RR) {
-      long long id = wrap(x0 - RR + tx, a.nx) + NX * yw + pz;
-      for (int c = 0; c < 3; ++c) sB[c][ty + RR][tx] = B[c * S + id];
-    } else if (tx >= 64 - RR) {
-      long long id = wrap(xg + RR, a.nx) + NX * yw + pz;
-      for (int c = 0; c < 3; ++c) sB[c][ty + RR][tx + 2 * RR] = B[c * S + id];
-    }
-    if (ty < RR) {
-      long long id = xw + NX * wrap(y0 - RR + ty, a.ny) + pz;
-      for (int c = 0; c < 3; ++c) sB[c][ty][tx + RR] = B[c * S + id];
-    } else if (ty >= TY - RR) {
-      long long id = xw + NX * wrap(yg + RR, a.ny) + pz;
-      for (int c = 0; c < 3; ++c) sB[c][ty + 2 * RR][tx + RR] = B[c * S + id];
-    }
-    __syncthreads();
+    lds_barrier();
     if (active) {
       R vx[Q], vy[Q];
-      // d/dx of B_y, B_z; d/dy of B_x, B_z; d/dz of B_x, B_y
 #pragma unroll
-      for (int k = 0; k < Q; ++k) vx[k] = sB[1][ty + RR][tx + k];
-      R dxBy = deriv<R, RR>(a.c, vx) / a.h[0];
+      for (int k = 0; k < Q; ++k) vx[k] = sB[buf][1][ty + RR][tx + k];
+      const R dxBy = deriv<R, RR>(a.c, vx) / a.h[0];
 #pragma unroll
-      for (int k = 0; k < Q; ++k) vx[k] = sB[2][ty + RR][tx + k];
-      R dxBz = deriv<R, RR>(a.c, vx) / a.h[0];
+      for (int k = 0; k < Q; ++k) vx[k] = sB[buf][2][ty + RR][tx + k];
+      const R dxBz = deriv<R, RR>(a.c, vx) / a.h[0];
 #pragma unroll
-      for (int k = 0; k < Q; ++k) vy[k] = sB[0][ty + k][tx + RR];
-      R dyBx = deriv<R, RR>(a.c, vy) / a.h[1];
+      for (int k = 0; k < Q; ++k) vy[k] = sB[buf][0][ty + k][tx + RR];
+      const R dyBx = deriv<R, RR>(a.c, vy) / a.h[1];
 #pragma unroll
-      for (int k = 0; k < Q; ++k) vy[k] = sB[2][ty + k][tx + RR];
-      R dyBz = deriv<R, RR>(a.c, vy) / a.h[1];
-      R dzBx = deriv<R, RR>(a.c, qx) / a.h[2];
-      R dzBy = deriv<R, RR>(a.c, qy) / a.h[2];
+      for (int k = 0; k < Q; ++k) vy[k] = sB[buf][2][ty + k][tx + RR];
+      const R dyBz = deriv<R, RR>(a.c, vy) / a.h[1];
+      const R dzBx = deriv<R, RR>(a.c, q[0]) / a.h[2];
+      const R dzBy = deriv<R, RR>(a.c, q[1]) / a.h[2];
       // J = curl B / rho  (SBP_operator.cl:126-135, kernel_time_integrator.cl:147)
-      R Jx = (dyBz - dzBy) / a.rho, Jy = (dzBx - dxBz) / a.rho, Jz = (dxBy - dyBx) / a.rho;
-      R Bx = qx[RR], By = qy[RR], Bz = sB[2][ty + RR][tx + RR];
+      const R Jx = (dyBz - dzBy) / a.rho, Jy = (dzBx - dxBz) / a.rho, Jz = (dxBy - dyBx) / a.rho;
+      const R Bx = q[0][RR], By = q[1][RR], Bz = q[2][RR];
+      const long long ic = col + NXY * z;
       a.Eout[ic] = Jy * Bz - Jz * By;
       a.Eout[S + ic] = Jz * Bx - Jx * Bz;
       a.Eout[2 * S + ic] = Jx * By - Jy * Bx;
     }
-    __syncthreads();
 #pragma unroll
-    for (int j = 0; j < Q - 1; ++j) {
-      qx[j] = qx[j + 1];
-      qy[j] = qy[j + 1];
-    }
+    for (int j = 0; j < Q + 1; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) q[c][j] = q[c][j + 1];
+  }
+}
+
+template <typename R, int RR, int TY>
+__global__ __launch_bounds__(64 * TY) void k_emf(FArgs<R> a) {
+  __shared__ R sB[2][3][TY + 2 * RR][64 + 2 * RR];
+  int x0, y0, z0;
+  tile_origin<TY>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
+  emf_tile<R, RR, TY>(a, x0, y0, z0, sB);
+}
+
+// Persistent batched form: blocks stride over a tile list covering every member.
+template <typename R, int RR, int TY>
+__global__ __launch_bounds__(64 * TY) void k_emf_batched(const FArgs<R>* __restrict__ args, const int4* __restrict__ tiles,
+                                                         int ntiles) {
+  __shared__ R sB[2][3][TY + 2 * RR][64 + 2 * RR];
+  const int chunk = (ntiles + kXcds - 1) / kXcds, slots = gridDim.x / kXcds;  // gridDim.x is a multiple of 8
+  const int first = (blockIdx.x % kXcds) * chunk, last = min(first + chunk, ntiles);
+  for (int t = first + (int)blockIdx.x / kXcds; t < last; t += slots) {
+    const int4 d = tiles[t];
+    emf_tile<R, RR, TY>(args[d.w], d.x, d.y, d.z, sB);
+    lds_barrier();
   }
 }
 
 // Stage kernel: RHS = convection + (-curl E) + dissipation, then the RK update.
-template <typename R, int RR, int TY, bool HALL, bool HOD>
-__global__ __launch_bounds__(64 * TY) void k_stage(FArgs<R> a) {
-  constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR, Q = 2 * RR + 1;
-  constexpr int NE = HALL ? 3 : 1, LYE = HALL ? LY : 1, LXE = HALL ? LX : 1;
-  __shared__ R sB[3][LY][LX];
-  __shared__ R sE[NE][LYE][LXE];
+// Same pipelining as k_emf, with B (3 components) and E (3 in LDS, x/y in the queue).
+#ifndef IEQ_STAGE_WAVES
+#define IEQ_STAGE_WAVES 2
+#endif
+template <typename R, int RR, int TY, bool HALL> struct StageLds {
+  static constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR;
+  static constexpr int NE = HALL ? 3 : 1, LYE = HALL ? LY : 1, LXE = HALL ? LX : 1;
+  R sB[2][3][LY][LX];
+  R sE[2][NE][LYE][LXE];
+};
+
+template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV>
+__device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, int z0, StageLds<R, RR, TY, HALL>& L) {
+  constexpr int Q = 2 * RR + 1;
+  auto& sB = L.sB;
+  auto& sE = L.sE;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * TY, z0 = blockIdx.z * a.zc;
-  const int xg = x0 + tx, yg = y0 + ty;
-  const bool active = xg < a.nx && yg < a.ny;
-  const int xw = wrap(xg, a.nx), yw = wrap(yg, a.ny);
+  const bool active = x0 + tx < a.nx && y0 + ty < a.ny;
   const long long S = a.S, NX = a.nx, NXY = (long long)a.nx * a.ny;
+  const long long col = wrap(x0 + tx, a.nx) + NX * wrap(y0 + ty, a.ny);
   const int zend = min(z0 + a.zc, a.nz);
   const R* B = a.Y;
   const R* E = a.E;
+  PlaneStage<R, RR, TY, 3> hsb;
+  PlaneStage<R, RR, TY, 3> hse;
+  hsb.init(tx, ty, x0, y0, a.nx, a.ny, NX);
+  if constexpr (HALL) hse.init(tx, ty, x0, y0, a.nx, a.ny, NX);
 
-  R qb[3][Q];
-  R qe[2][HALL ? Q : 1];
+  R qb[3][Q + 2];
+  R qe[2][HALL ? Q + 2 : 1];
+  R ez = R(0);  // E_z of the own column at the plane being staged
 #pragma unroll
-  for (int j = 0; j < Q - 1; ++j) {
-    long long id = xw + NX * yw + NXY * wrap(z0 - RR + j, a.nz);
+  for (int j = 0; j < Q + 1; ++j) {
+    const long long id = col + NXY * wrap(z0 - RR + j, a.nz);
 #pragma unroll
     for (int c = 0; c < 3; ++c) qb[c][j] = B[c * S + id];
     if constexpr (HALL) {
@@ -266,121 +373,130 @@ __global__ __launch_bounds__(64 * TY) void k_stage(FArgs<R> a) {
       qe[1][j] = E[S + id];
     }
   }
+  hsb.load(B, S, NXY * z0);
+  if constexpr (HALL) {
+    hse.load(E, S, NXY * z0);
+    ez = E[2 * S + col + NXY * z0];
+  }
   for (int z = z0; z < zend; ++z) {
-    {
-      long long id = xw + NX * yw + NXY * wrap(z + RR, a.nz);
+    const int buf = (z - z0) & 1;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) qb[c][Q - 1] = B[c * S + id];
-      if constexpr (HALL) {
-        qe[0][Q - 1] = E[id];
-        qe[1][Q - 1] = E[S + id];
-      }
-    }
-    const long long pz = NXY * z;
-    const long long ic = xw + NX * yw + pz;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) sB[c][ty + RR][tx + RR] = qb[c][RR];
+    for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = qb[c][RR];
+    hsb.store(sB[buf], tx, ty);
     if constexpr (HALL) {
-      sE[0][ty + RR][tx + RR] = qe[0][RR];
-      sE[1][ty + RR][tx + RR] = qe[1][RR];
-      sE[2][ty + RR][tx + RR] = E[2 * S + ic];
+      sE[buf][0][ty + RR][tx + RR] = qe[0][RR];
+      sE[buf][1][ty + RR][tx + RR] = qe[1][RR];
+      sE[buf][2][ty + RR][tx + RR] = ez;
+      hse.store(sE[buf], tx, ty);
     }
-    if (tx < RR) {
-      long long id = wrap(x0 - RR + tx, a.nx) + NX * yw + pz;
-      for (int c = 0; c < 3; ++c) sB[c][ty + RR][tx] = B[c * S + id];
-      if constexpr (HALL)
-        for (int c = 1; c < 3; ++c) sE[c][ty + RR][tx] = E[c * S + id];
-    } else if (tx >= 64 - RR) {
-      long long id = wrap(xg + RR, a.nx) + NX * yw + pz;
-      for (int c = 0; c < 3; ++c) sB[c][ty + RR][tx + 2 * RR] = B[c * S + id];
-      if constexpr (HALL)
-        for (int c = 1; c < 3; ++c) sE[c][ty + RR][tx + 2 * RR] = E[c * S + id];
-    }
-    if (ty < RR) {
-      long long id = xw + NX * wrap(y0 - RR + ty, a.ny) + pz;
-      for (int c = 0; c < 3; ++c) sB[c][ty][tx + RR] = B[c * S + id];
+    {  // prefetch: column plane z+RR+2, halo plane z+1
+      const long long id = col + NXY * wrap(z + RR + 2, a.nz);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) qb[c][Q + 1] = B[c * S + id];
       if constexpr (HALL) {
-        sE[0][ty][tx + RR] = E[id];
-        sE[2][ty][tx + RR] = E[2 * S + id];
+        qe[0][Q + 1] = E[id];
+        qe[1][Q + 1] = E[S + id];
       }
-    } else if (ty >= TY - RR) {
-      long long id = xw + NX * wrap(yg + RR, a.ny) + pz;
-      for (int c = 0; c < 3; ++c) sB[c][ty + 2 * RR][tx + RR] = B[c * S + id];
-      if constexpr (HALL) {
-        sE[0][ty + 2 * RR][tx + RR] = E[id];
-        sE[2][ty + 2 * RR][tx + RR] = E[2 * S + id];
+      if (z + 1 < zend) {
+        hsb.load(B, S, NXY * (z + 1));
+        if constexpr (HALL) {
+          hse.load(E, S, NXY * (z + 1));
+          ez = E[2 * S + col + NXY * (z + 1)];
+        }
       }
     }
-    __syncthreads();
+    const long long ic = col + NXY * z;
+    R pv[3], qv[3];  // pointwise RK operands, loaded before the barrier
     if (active) {
-      R rhs[3] = {R(0), R(0), R(0)};
-      R hod[3] = {R(0), R(0), R(0)};
-      R der[3][3];  // der[d][c] = sum_i D_i B_c along d (no 1/dx)
-      R v[Q];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-#pragma unroll
-        for (int k = 0; k < Q; ++k) v[k] = sB[c][ty + RR][tx + k];
-        if constexpr (HOD) hod[c] = hodsum<R, RR>(a.c, v);
-        der[0][c] = a.conv ? deriv<R, RR>(a.c, v) : R(0);
-#pragma unroll
-        for (int k = 0; k < Q; ++k) v[k] = sB[c][ty + k][tx + RR];
-        if constexpr (HOD) hod[c] = hod[c] + hodsum<R, RR>(a.c, v);
-        der[1][c] = a.conv ? deriv<R, RR>(a.c, v) : R(0);
-        if constexpr (HOD) hod[c] = hod[c] + hodsum<R, RR>(a.c, qb[c]);
-        der[2][c] = a.conv ? deriv<R, RR>(a.c, qb[c]) : R(0);
+        pv[c] = a.mode == 0 ? (a.P ? a.P[c * S + ic] : R(0)) : a.b2[c * S + ic];
+        qv[c] = (a.mode == 0 && a.Q) ? a.Q[c * S + ic] : R(0);
       }
-      if (a.conv) {
+    }
+    lds_barrier();
+    if (active) {
+      R rhs[3] = {R(0), R(0), R(0)};
+      if constexpr (CONV) {
+        // uniform-velocity convection: sum_d (2/dx_d) 0.5 ([ZERO] u (D_d B)_d - u_d (D_d B))
+        R v[Q];
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
+        for (int d = 0; d < 3; ++d) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
-            rhs[c] = rhs[c] + a.two_h[d] * (R(0.5) * ((a.src_zero ? a.u[c] * der[d][d] : R(0)) - a.u[d] * der[d][c]));
+          for (int c = 0; c < 3; ++c) {
+            if (d == 0) {
+#pragma unroll
+              for (int k = 0; k < Q; ++k) v[k] = sB[buf][c][ty + RR][tx + k];
+            } else if (d == 1) {
+#pragma unroll
+              for (int k = 0; k < Q; ++k) v[k] = sB[buf][c][ty + k][tx + RR];
+            } else {
+#pragma unroll
+              for (int k = 0; k < Q; ++k) v[k] = qb[c][k];
+            }
+            const R der = deriv<R, RR>(a.c, v);
+            const R f = a.two_h[d] * R(0.5);
+            rhs[c] = rhs[c] - f * (a.u[d] * der);
+            if (c == d && a.src_zero) {
+#pragma unroll
+              for (int e = 0; e < 3; ++e) rhs[e] = rhs[e] + f * (a.u[e] * der);
+            }
+          }
+        }
       }
       if constexpr (HALL) {
         R ev[Q];
 #pragma unroll
-        for (int k = 0; k < Q; ++k) ev[k] = sE[1][ty + RR][tx + k];
-        R dxEy = deriv<R, RR>(a.c, ev);
+        for (int k = 0; k < Q; ++k) ev[k] = sE[buf][1][ty + RR][tx + k];
+        const R dxEy = deriv<R, RR>(a.c, ev);
 #pragma unroll
-        for (int k = 0; k < Q; ++k) ev[k] = sE[2][ty + RR][tx + k];
-        R dxEz = deriv<R, RR>(a.c, ev);
+        for (int k = 0; k < Q; ++k) ev[k] = sE[buf][2][ty + RR][tx + k];
+        const R dxEz = deriv<R, RR>(a.c, ev);
 #pragma unroll
-        for (int k = 0; k < Q; ++k) ev[k] = sE[0][ty + k][tx + RR];
-        R dyEx = deriv<R, RR>(a.c, ev);
+        for (int k = 0; k < Q; ++k) ev[k] = sE[buf][0][ty + k][tx + RR];
+        const R dyEx = deriv<R, RR>(a.c, ev);
 #pragma unroll
-        for (int k = 0; k < Q; ++k) ev[k] = sE[2][ty + k][tx + RR];
-        R dyEz = deriv<R, RR>(a.c, ev);
-        R dzEx = deriv<R, RR>(a.c, qe[0]);
-        R dzEy = deriv<R, RR>(a.c, qe[1]);
+        for (int k = 0; k < Q; ++k) ev[k] = sE[buf][2][ty + k][tx + RR];
+        const R dyEz = deriv<R, RR>(a.c, ev);
+        const R dzEx = deriv<R, RR>(a.c, qe[0]);
+        const R dzEy = deriv<R, RR>(a.c, qe[1]);
         rhs[0] = rhs[0] + (a.inv_h[1] * (-dyEz) + a.inv_h[2] * dzEy);
         rhs[1] = rhs[1] + (a.inv_h[0] * dxEz + a.inv_h[2] * (-dzEx));
         rhs[2] = rhs[2] + (a.inv_h[0] * (-dxEy) + a.inv_h[1] * dyEx);
       }
       if constexpr (HOD) {
+        R v[Q];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) rhs[c] = rhs[c] + hod[c];
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+          for (int k = 0; k < Q; ++k) v[k] = sB[buf][c][ty + RR][tx + k];
+          R h = hodsum<R, RR>(a.c, v);
+#pragma unroll
+          for (int k = 0; k < Q; ++k) v[k] = sB[buf][c][ty + k][tx + RR];
+          h = h + hodsum<R, RR>(a.c, v);
+          h = h + hodsum<R, RR>(a.c, qb[c]);
+          rhs[c] = rhs[c] + h;
+        }
       }
       if (a.mode == 0) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           R r = a.cD * rhs[c];
-          if (a.Q) r = a.cQ * a.Q[c * S + ic] + r;
-          if (a.P) r = a.cP * a.P[c * S + ic] + r;
+          if (a.Q) r = a.cQ * qv[c] + r;
+          if (a.P) r = a.cP * pv[c] + r;
           a.out[c * S + ic] = r;
         }
       } else {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          R b2 = a.cP * a.b2[c * S + ic] + a.cD * rhs[c];
+          const R b2 = a.cP * pv[c] + a.cD * rhs[c];
           a.b2[c * S + ic] = b2;
           a.out[c * S + ic] = qb[c][RR] + a.cQ * b2;
         }
       }
     }
-    __syncthreads();
 #pragma unroll
-    for (int j = 0; j < Q - 1; ++j) {
+    for (int j = 0; j < Q + 1; ++j) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) qb[c][j] = qb[c][j + 1];
       if constexpr (HALL) {
@@ -388,6 +504,27 @@ __global__ __launch_bounds__(64 * TY) void k_stage(FArgs<R> a) {
         qe[1][j] = qe[1][j + 1];
       }
     }
+  }
+}
+
+template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV>
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(IEQ_STAGE_WAVES, 8))) void k_stage(FArgs<R> a) {
+  __shared__ StageLds<R, RR, TY, HALL> L;
+  int x0, y0, z0;
+  tile_origin<TY>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
+  stage_tile<R, RR, TY, HALL, HOD, CONV>(a, x0, y0, z0, L);
+}
+
+template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV>
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(IEQ_STAGE_WAVES, 8))) void k_stage_batched(
+    const FArgs<R>* __restrict__ args, const int4* __restrict__ tiles, int ntiles) {
+  __shared__ StageLds<R, RR, TY, HALL> L;
+  const int chunk = (ntiles + kXcds - 1) / kXcds, slots = gridDim.x / kXcds;  // gridDim.x is a multiple of 8
+  const int first = (blockIdx.x % kXcds) * chunk, last = min(first + chunk, ntiles);
+  for (int t = first + (int)blockIdx.x / kXcds; t < last; t += slots) {
+    const int4 d = tiles[t];
+    stage_tile<R, RR, TY, HALL, HOD, CONV>(args[d.w], d.x, d.y, d.z, L);
+    lds_barrier();
   }
 }
 
@@ -432,33 +569,55 @@ template <typename R> __global__ void k_advance_time(R* time, R dt, long long ns
   }
 }
 
-constexpr int kTY = 4;
-
-template <typename R, int RR>
-void launch_emf(const FArgs<R>& a, hipStream_t s) {
-  dim3 grid((a.nx + 63) / 64, (a.ny + kTY - 1) / kTY, (a.nz + a.zc - 1) / a.zc);
-  k_emf<R, RR, kTY><<<grid, dim3(64, kTY), 0, s>>>(a);
+template <typename R, int RR, int TY>
+void launch_emf_ty(const FArgs<R>& a, hipStream_t s) {
+  const int grid = ((a.nx + 63) / 64) * ((a.ny + TY - 1) / TY) * ((a.nz + a.zc - 1) / a.zc);
+  k_emf<R, RR, TY><<<grid, dim3(64, TY), 0, s>>>(a);
 }
 
-template <typename R, int RR>
-void launch_stage(const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
-  dim3 grid((a.nx + 63) / 64, (a.ny + kTY - 1) / kTY, (a.nz + a.zc - 1) / a.zc);
-  dim3 blk(64, kTY);
-  if (hall && hod) k_stage<R, RR, kTY, true, true><<<grid, blk, 0, s>>>(a);
-  else if (hall) k_stage<R, RR, kTY, true, false><<<grid, blk, 0, s>>>(a);
-  else if (hod) k_stage<R, RR, kTY, false, true><<<grid, blk, 0, s>>>(a);
-  else k_stage<R, RR, kTY, false, false><<<grid, blk, 0, s>>>(a);
+template <typename R, int RR, int TY>
+void launch_stage_ty(const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
+  const int grid = ((a.nx + 63) / 64) * ((a.ny + TY - 1) / TY) * ((a.nz + a.zc - 1) / a.zc);
+  dim3 blk(64, TY);
+  if (a.conv) {
+    if (hall && hod) k_stage<R, RR, TY, true, true, true><<<grid, blk, 0, s>>>(a);
+    else if (hall) k_stage<R, RR, TY, true, false, true><<<grid, blk, 0, s>>>(a);
+    else if (hod) k_stage<R, RR, TY, false, true, true><<<grid, blk, 0, s>>>(a);
+    else k_stage<R, RR, TY, false, false, true><<<grid, blk, 0, s>>>(a);
+  } else {
+    if (hall && hod) k_stage<R, RR, TY, true, true, false><<<grid, blk, 0, s>>>(a);
+    else if (hall) k_stage<R, RR, TY, true, false, false><<<grid, blk, 0, s>>>(a);
+    else if (hod) k_stage<R, RR, TY, false, true, false><<<grid, blk, 0, s>>>(a);
+    else k_stage<R, RR, TY, false, false, false><<<grid, blk, 0, s>>>(a);
+  }
 }
 
-template <typename R> void emf(int rr, const FArgs<R>& a, hipStream_t s) {
-  if (rr == 1) launch_emf<R, 1>(a, s);
-  else if (rr == 2) launch_emf<R, 2>(a, s);
-  else launch_emf<R, 3>(a, s);
+template <typename R> void emf(int rr, int ty, const FArgs<R>& a, hipStream_t s) {
+  if (ty == 8) {
+    if (rr == 1) launch_emf_ty<R, 1, 8>(a, s);
+    else if (rr == 2) launch_emf_ty<R, 2, 8>(a, s);
+    else launch_emf_ty<R, 3, 8>(a, s);
+  } else {
+    if (rr == 1) launch_emf_ty<R, 1, 4>(a, s);
+    else if (rr == 2) launch_emf_ty<R, 2, 4>(a, s);
+    else launch_emf_ty<R, 3, 4>(a, s);
+  }
 }
-template <typename R> void stage(int rr, const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
-  if (rr == 1) launch_stage<R, 1>(a, hall, hod, s);
-  else if (rr == 2) launch_stage<R, 2>(a, hall, hod, s);
-  else launch_stage<R, 3>(a, hall, hod, s);
+template <typename R> void stage(int rr, int ty, const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
+  if (ty == 8) {
+    if (rr == 1) launch_stage_ty<R, 1, 8>(a, hall, hod, s);
+    else if (rr == 2) launch_stage_ty<R, 2, 8>(a, hall, hod, s);
+    else launch_stage_ty<R, 3, 8>(a, hall, hod, s);
+  } else {
+    if (rr == 1) launch_stage_ty<R, 1, 4>(a, hall, hod, s);
+    else if (rr == 2) launch_stage_ty<R, 2, 4>(a, hall, hod, s);
+    else launch_stage_ty<R, 3, 4>(a, hall, hod, s);
+  }
+}
+
+int env_int(const char* k, int dflt) {
+  const char* v = std::getenv(k);
+  return v && *v ? std::atoi(v) : dflt;
 }
 
 // Central-row coefficients of the program's operator set (host copies of the tables).
@@ -492,6 +651,27 @@ HostOp host_op(int ops) {
     default: return {IEQ_6TH_EXT_NB, IEQ_6TH_EXT_W, IEQ_6TH_EXT_W_HOD, IEQ_6TH_EXT_ORDER, T::ieq_6th_ext_diff, T::ieq_6th_ext_minv, T::ieq_6th_ext_dho};
   }
 }
+
+}  // namespace
+
+const double* host_lap_table(int opset, int lap, int* nbl, int* wl) {
+  namespace T = ieq_host_tables;
+  struct L { const double* t; int nb, w; };
+  static const L tabs[6][3] = {
+      {{T::ieq_2nd_lap_ws_d0, IEQ_2ND_LAP_WS_D0_NB, IEQ_2ND_LAP_WS_D0_W}, {T::ieq_2nd_lap_ns_d0, IEQ_2ND_LAP_NS_D0_NB, IEQ_2ND_LAP_NS_D0_W}, {T::ieq_2nd_lap_ws_ln, IEQ_2ND_LAP_WS_LN_NB, IEQ_2ND_LAP_WS_LN_W}},
+      {{T::ieq_4th_lap_ws_d0, IEQ_4TH_LAP_WS_D0_NB, IEQ_4TH_LAP_WS_D0_W}, {T::ieq_4th_lap_ns_d0, IEQ_4TH_LAP_NS_D0_NB, IEQ_4TH_LAP_NS_D0_W}, {T::ieq_4th_lap_ws_ln, IEQ_4TH_LAP_WS_LN_NB, IEQ_4TH_LAP_WS_LN_W}},
+      {{T::ieq_6th_lap_ws_d0, IEQ_6TH_LAP_WS_D0_NB, IEQ_6TH_LAP_WS_D0_W}, {T::ieq_6th_lap_ns_d0, IEQ_6TH_LAP_NS_D0_NB, IEQ_6TH_LAP_NS_D0_W}, {T::ieq_6th_lap_ws_ln, IEQ_6TH_LAP_WS_LN_NB, IEQ_6TH_LAP_WS_LN_W}},
+      {{T::ieq_2nd_ext_lap_ws_d0, IEQ_2ND_EXT_LAP_WS_D0_NB, IEQ_2ND_EXT_LAP_WS_D0_W}, {nullptr, 0, 0}, {T::ieq_2nd_ext_lap_ws_ln, IEQ_2ND_EXT_LAP_WS_LN_NB, IEQ_2ND_EXT_LAP_WS_LN_W}},
+      {{T::ieq_4th_ext_lap_ws_d0, IEQ_4TH_EXT_LAP_WS_D0_NB, IEQ_4TH_EXT_LAP_WS_D0_W}, {nullptr, 0, 0}, {T::ieq_4th_ext_lap_ws_ln, IEQ_4TH_EXT_LAP_WS_LN_NB, IEQ_4TH_EXT_LAP_WS_LN_W}},
+      {{T::ieq_6th_ext_lap_ws_d0, IEQ_6TH_EXT_LAP_WS_D0_NB, IEQ_6TH_EXT_LAP_WS_D0_W}, {nullptr, 0, 0}, {T::ieq_6th_ext_lap_ws_ln, IEQ_6TH_EXT_LAP_WS_LN_NB, IEQ_6TH_EXT_LAP_WS_LN_W}},
+  };
+  if (opset < 0 || opset > 5 || lap < 0 || lap > 2) return nullptr;
+  *nbl = tabs[opset][lap].nb;
+  *wl = tabs[opset][lap].w;
+  return tabs[opset][lap].t;
+}
+
+namespace {
 
 // Stencil half-width and coefficients of the central rows; returns -1 when the
 // central row has non-zero entries beyond the dissipation half-width.
@@ -645,7 +825,9 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
   a.nx = p.nx;
   a.ny = p.ny;
   a.nz = p.nz;
-  a.zc = p.nz >= 64 ? 16 : 8;
+  a.zc = env_int("IEQ_FUSED_ZC", 8);
+  if (a.zc < 1) a.zc = 1;
+  const int ty = env_int("IEQ_FUSED_TY", 8) == 4 ? 4 : 8;
   a.S = info.stride;
   a.c = coef;
   const double h[3] = {p.dx, p.dy, p.dz};
@@ -672,7 +854,7 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
       if (st.kind == 3) {  // EMF of the field the next stage differentiates
         a.Y = st.Y == 0 ? b1 : F[st.Y];
         a.Eout = J;
-        emf<R>(rr, a, s);
+        emf<R>(rr, ty, a, s);
       } else if (st.kind == 0) {
         R* FF[3] = {b1, F[1], F[2]};
         a.Y = FF[st.Y];
@@ -684,7 +866,7 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
         a.cP = (R)st.cP;
         a.cQ = (R)st.cQ;
         a.cD = (R)(st.cD * p.dt);
-        stage<R>(rr, a, p.hall, hod, s);
+        stage<R>(rr, ty, a, p.hall, hod, s);
       } else if (st.kind == 1) {
         uint32_t g[3] = {(uint32_t)p.nx, (uint32_t)p.ny, (uint32_t)p.nz};
         R* FF[3] = {b1, F[1], F[2]};
@@ -701,7 +883,7 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
         a.cP = (R)st.cP;
         a.cQ = (R)st.cQ;
         a.cD = DT;
-        stage<R>(rr, a, p.hall, hod, s);
+        stage<R>(rr, ty, a, p.hall, hod, s);
         R* t = b1;
         b1 = b1alt;
         b1alt = t;
@@ -719,6 +901,363 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
 }
 
 }  // namespace
+
+
+// ---------------------------------------------------------------------------------
+// Ensembles: many independent periodic solves stepped together.  Every fused kernel of
+// an RK stage is one persistent launch whose blocks stride over a tile list of all
+// members; each tile carries its member index and the member's stage arguments are an
+// FArgs entry in device memory.  One step (two for the 2N schemes, whose ping-pong
+// register alternates) is captured once in a hipGraph and replayed.
+
+struct Ensemble {
+  int real_bytes = 8;
+  int rr = 3, hall = 0, hod = 0, conv = 0, src_zero = 0;
+  StencilCoef<double> coef;  // central-row coefficients (converted to REAL per member)
+  std::vector<ieq_member> m;
+  std::vector<StageSpec> plan;
+  int nf = 0, steps_per_graph = 1;
+  std::vector<std::vector<void*>> b1_at;  // per unrolled stage: b1 pointers after swaps (2N)
+  void* dargs = nullptr;                  // FArgs<R>[nstage_unrolled][nmembers]
+  size_t args_bytes = 0;
+  int4* dtiles = nullptr;
+  int ntiles = 0, ty = 8, zc = 8, order = 0, grid_emf = 0, grid_stage = 0;
+  bool tuned = false;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int ncu = 256;
+  bool final_in_alt = false;  // 2N with an odd number of stages per graph
+};
+
+namespace {
+
+template <typename R> FArgs<R> member_base(const Ensemble& e, const ieq_member& mb) {
+  FArgs<R> a{};
+  a.nx = mb.nx;
+  a.ny = mb.ny;
+  a.nz = mb.nz;
+  a.zc = e.zc;
+  a.S = (((long long)mb.nx * mb.ny * mb.nz) + 63) / 64 * 64;
+  for (int k = 0; k < 4; ++k) {
+    a.c.dp[k] = (R)e.coef.dp[k];
+    a.c.dm[k] = (R)e.coef.dm[k];
+    a.c.w[k] = (R)e.coef.w[k];
+  }
+  const double h[3] = {mb.dx, mb.dy, mb.dz};
+  for (int d = 0; d < 3; ++d) {
+    a.inv_h[d] = (R)(1.0 / h[d]);
+    a.two_h[d] = (R)(2.0 / h[d]);
+    a.h[d] = (R)h[d];
+  }
+  a.rho = (R)mb.rho;
+  a.u[0] = (R)mb.ux;
+  a.u[1] = (R)mb.uy;
+  a.u[2] = (R)mb.uz;
+  a.conv = e.conv;
+  a.src_zero = e.src_zero;
+  return a;
+}
+
+// FArgs of every unrolled stage entry for every member (zc taken from e.zc).
+template <typename R> std::vector<FArgs<R>> ensemble_args(Ensemble& e) {
+  const int nm = (int)e.m.size();
+  std::vector<FArgs<R>> out;
+  std::vector<R*> b1(nm), b1alt(nm);
+  for (int i = 0; i < nm; ++i) {
+    b1[i] = (R*)e.m[i].b1;
+    b1alt[i] = (R*)e.m[i].b3;
+  }
+  for (int step = 0; step < e.steps_per_graph; ++step)
+    for (const StageSpec& st : e.plan) {
+      for (int i = 0; i < nm; ++i) {
+        const ieq_member& mb = e.m[i];
+        R* F[3] = {b1[i], (R*)mb.b2, (R*)mb.b3};
+        FArgs<R> a = member_base<R>(e, mb);
+        if (st.kind == 3) {
+          a.Y = st.Y == 0 ? b1[i] : F[st.Y];
+          a.Eout = (R*)mb.emf;
+        } else if (st.kind == 0) {
+          a.Y = F[st.Y];
+          a.E = (R*)mb.emf;
+          a.mode = 0;
+          a.out = F[st.out];
+          a.P = st.P >= 0 ? F[st.P] : nullptr;
+          a.Q = st.Q >= 0 ? F[st.Q] : nullptr;
+          a.cP = (R)st.cP;
+          a.cQ = (R)st.cQ;
+          a.cD = (R)(st.cD * mb.dt);
+        } else {  // 2N
+          a.Y = b1[i];
+          a.E = (R*)mb.emf;
+          a.mode = 1;
+          a.b2 = (R*)mb.b2;
+          a.out = b1alt[i];
+          a.cP = (R)st.cP;
+          a.cQ = (R)st.cQ;
+          a.cD = (R)mb.dt;
+        }
+        out.push_back(a);
+      }
+      if (st.kind == 4)
+        for (int i = 0; i < nm; ++i) std::swap(b1[i], b1alt[i]);
+    }
+  e.final_in_alt = b1[0] != (R*)e.m[0].b1;
+  return out;
+}
+
+std::vector<int4> ensemble_tiles(const Ensemble& e, int ty, int zc, int order) {
+  std::vector<int> idx(e.m.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+  if (order == 1)
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+      return (long long)e.m[a].nx * e.m[a].ny * e.m[a].nz > (long long)e.m[b].nx * e.m[b].ny * e.m[b].nz;
+    });
+  std::vector<int4> t;
+  for (int i : idx) {
+    const ieq_member& mb = e.m[i];
+    for (int z = 0; z < mb.nz; z += zc)
+      for (int y = 0; y < mb.ny; y += ty)
+        for (int x = 0; x < mb.nx; x += 64) t.push_back(make_int4(x, y, z, i));
+  }
+  return t;
+}
+
+template <typename R, int RR, int TY> const void* ens_kernel_t(const Ensemble& e, int kind) {
+  if (kind == 3) return reinterpret_cast<const void*>(&k_emf_batched<R, RR, TY>);
+  const bool h = e.hall, d = e.hod;
+  if (e.conv) {
+    if (h && d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, true, true>);
+    if (h) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, false, true>);
+    if (d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, true, true>);
+    return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, false, true>);
+  }
+  if (h && d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, true, false>);
+  if (h) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, false, false>);
+  if (d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, true, false>);
+  return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, false, false>);
+}
+
+// Kernel of an entry (kind 3: EMF, otherwise the stage kernel) for the ensemble's settings.
+template <typename R> const void* ens_kernel(const Ensemble& e, int ty, int kind) {
+  if (ty == 4) {
+    if (e.rr == 1) return ens_kernel_t<R, 1, 4>(e, kind);
+    if (e.rr == 2) return ens_kernel_t<R, 2, 4>(e, kind);
+    return ens_kernel_t<R, 3, 4>(e, kind);
+  }
+  if (e.rr == 1) return ens_kernel_t<R, 1, 8>(e, kind);
+  if (e.rr == 2) return ens_kernel_t<R, 2, 8>(e, kind);
+  return ens_kernel_t<R, 3, 8>(e, kind);
+}
+
+template <typename R> hipError_t ens_launch_steps(Ensemble& e, hipStream_t s) {
+  const FArgs<R>* base = (const FArgs<R>*)e.dargs;
+  const size_t nm = e.m.size();
+  size_t k = 0;
+  for (int step = 0; step < e.steps_per_graph; ++step)
+    for (const StageSpec& st : e.plan) {
+      const FArgs<R>* a = base + k * nm;
+      ++k;
+      void* kargs[3] = {(void*)&a, (void*)&e.dtiles, (void*)&e.ntiles};
+      const int grid = st.kind == 3 ? e.grid_emf : e.grid_stage;
+      hipError_t r = hipLaunchKernel(ens_kernel<R>(e, e.ty, st.kind), dim3(grid), dim3(64, e.ty), kargs, 0, s);
+      if (r != hipSuccess) return r;
+    }
+  return hipSuccess;
+}
+
+template <typename R> int ens_configure(Ensemble& e, int ty, int zc, int order, std::string& err) {
+  e.ty = ty;
+  e.zc = zc;
+  e.order = order;
+  std::vector<FArgs<R>> args = ensemble_args<R>(e);
+  const size_t ab = args.size() * sizeof(FArgs<R>);
+  if (e.args_bytes < ab) {
+    if (e.dargs) (void)hipFree(e.dargs);
+    e.dargs = nullptr;
+    if (hipMalloc(&e.dargs, ab) != hipSuccess) return err = "hipMalloc failed", -4;
+    e.args_bytes = ab;
+  }
+  if (hipMemcpyAsync(e.dargs, args.data(), ab, hipMemcpyHostToDevice, e.stream) != hipSuccess) return err = "copy failed", -4;
+  std::vector<int4> tiles = ensemble_tiles(e, ty, zc, order);
+  if (e.dtiles) (void)hipFree(e.dtiles);
+  e.dtiles = nullptr;
+  if (hipMalloc(&e.dtiles, tiles.size() * sizeof(int4)) != hipSuccess) return err = "hipMalloc failed", -4;
+  if (hipMemcpyAsync(e.dtiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, e.stream) != hipSuccess)
+    return err = "copy failed", -4;
+  e.ntiles = (int)tiles.size();
+  // persistent grids: as many blocks as can be resident at once (registers and LDS bound it)
+  int occ_emf = 1, occ_stage = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_emf, ens_kernel<R>(e, ty, 3), 64 * ty, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_stage, ens_kernel<R>(e, ty, 0), 64 * ty, 0) != hipSuccess)
+    return err = "occupancy query failed", -4;
+  // multiples of 8 (one equal share of resident slots per XCD), no more than the tiles need
+  const int need = (e.ntiles + 7) / 8 * 8;
+  e.grid_emf = std::min(need, (e.ncu * std::max(1, occ_emf) + 7) / 8 * 8);
+  e.grid_stage = std::min(need, (e.ncu * std::max(1, occ_stage) + 7) / 8 * 8);
+  if (hipStreamSynchronize(e.stream) != hipSuccess) return err = "sync failed", -4;
+  return 0;
+}
+
+template <typename R> int ens_capture(Ensemble& e, std::string& err) {
+  if (e.exec) (void)hipGraphExecDestroy(e.exec);
+  if (e.graph) (void)hipGraphDestroy(e.graph);
+  e.exec = nullptr;
+  e.graph = nullptr;
+  if (hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return err = "capture begin failed", -4;
+  hipError_t r = ens_launch_steps<R>(e, e.stream);
+  if (hipStreamEndCapture(e.stream, &e.graph) != hipSuccess || r != hipSuccess) return err = "capture failed", -4;
+  if (hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0) != hipSuccess) return err = "graph instantiate failed", -4;
+  return 0;
+}
+
+size_t member_bytes(const Ensemble& e, const ieq_member& mb) {
+  return (size_t)3 * ((((size_t)mb.nx * mb.ny * mb.nz) + 63) / 64 * 64) * e.real_bytes;
+}
+
+template <typename R> int ens_tune(Ensemble& e, std::string& err) {
+  // save the registers the trial steps overwrite
+  size_t total = 0;
+  for (auto& mb : e.m) total += 3 * member_bytes(e, mb);
+  char* save = nullptr;
+  if (hipMalloc(&save, total) != hipSuccess) return err = "hipMalloc failed (tuning snapshot)", -4;
+  size_t off = 0;
+  for (auto& mb : e.m) {
+    void* f[3] = {mb.b1, mb.b2, mb.b3};
+    for (int j = 0; j < 3; ++j) {
+      (void)hipMemcpyAsync(save + off, f[j], member_bytes(e, mb), hipMemcpyDeviceToDevice, e.stream);
+      off += member_bytes(e, mb);
+    }
+  }
+  float best = 1e30f;
+  int bty = 8, bzc = 8, bord = 1;
+  hipEvent_t t0, t1;
+  (void)hipEventCreate(&t0);
+  (void)hipEventCreate(&t1);
+  const int tys[2] = {4, 8}, zcs[3] = {4, 8, 16};
+  for (int ty : tys)
+    for (int zc : zcs)
+      for (int ord = 0; ord < 2; ++ord) {
+        int rc = ens_configure<R>(e, ty, zc, ord, err);
+        if (rc) return rc;
+        hipError_t r = ens_launch_steps<R>(e, e.stream);  // warm-up
+        (void)hipEventRecord(t0, e.stream);
+        if (r == hipSuccess) r = ens_launch_steps<R>(e, e.stream);
+        if (r == hipSuccess) r = ens_launch_steps<R>(e, e.stream);
+        (void)hipEventRecord(t1, e.stream);
+        (void)hipEventSynchronize(t1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        if (r != hipSuccess) return err = std::string("tuning launch failed: ") + hipGetErrorString(r), -4;
+        if (ms < best) {
+          best = ms;
+          bty = ty;
+          bzc = zc;
+          bord = ord;
+        }
+      }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  off = 0;
+  for (auto& mb : e.m) {
+    void* f[3] = {mb.b1, mb.b2, mb.b3};
+    for (int j = 0; j < 3; ++j) {
+      (void)hipMemcpyAsync(f[j], save + off, member_bytes(e, mb), hipMemcpyDeviceToDevice, e.stream);
+      off += member_bytes(e, mb);
+    }
+  }
+  (void)hipStreamSynchronize(e.stream);
+  (void)hipFree(save);
+  int rc = ens_configure<R>(e, bty, bzc, bord, err);
+  if (rc) return rc;
+  e.tuned = true;
+  return ens_capture<R>(e, err);
+}
+
+}  // namespace
+
+int ensemble_create(const Params& p, const ieq_program_info& info, const ieq_member* members, int n, const char* integrator,
+                    Ensemble** out, std::string& err) {
+  *out = nullptr;
+  if (!p.periodic) return err = "ensembles need a periodic program", IEQ_ERR_UNSUPPORTED;
+  if (p.diss != 0 && p.diss != 1) return err = "ensembles support high-order dissipation only", IEQ_ERR_UNSUPPORTED;
+  std::string I(integrator);
+  if (I != "SSPRK33" && I != "CarpenterKennedy2N54" && I != "ToulorgeDesmet2N84F")
+    return err = "ensembles support SSPRK33, CarpenterKennedy2N54 and ToulorgeDesmet2N84F", IEQ_ERR_UNSUPPORTED;
+  Ensemble* e = new Ensemble();
+  e->real_bytes = info.real_bytes;
+  if (!fused_plan(I, p.hall, e->plan, e->nf)) {
+    delete e;
+    return err = "unknown integrator", IEQ_ERR_ARGS;
+  }
+  e->rr = central_coefs<double>(info.opset, p.ho_factor, e->coef);
+  if (e->rr < 1 || e->rr > 3) {
+    delete e;
+    return err = "ensembles need an operator whose central rows are at most 7 wide", IEQ_ERR_UNSUPPORTED;
+  }
+  e->hall = p.hall;
+  e->hod = p.diss == 1;
+  e->src_zero = p.form_source == 0;
+  for (int i = 0; i < n; ++i) {
+    const ieq_member& mb = members[i];
+    if (mb.nx < 1 || mb.ny < 1 || mb.nz < 1 || !mb.b1 || !mb.b2 || !mb.b3 || !mb.emf || !(mb.rho != 0.0) || !(mb.dx > 0) ||
+        !(mb.dy > 0) || !(mb.dz > 0)) {
+      delete e;
+      return err = "invalid member " + std::to_string(i), IEQ_ERR_ARGS;
+    }
+    if (mb.ux != 0.0 || mb.uy != 0.0 || mb.uz != 0.0) e->conv = 1;
+    e->m.push_back(mb);
+  }
+  e->steps_per_graph = (e->plan.size() > 0 && e->plan.back().kind == 4 && (e->plan.size() / (p.hall ? 2 : 1)) % 2) ? 2 : 1;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) e->ncu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return err = "stream creation failed", IEQ_ERR_HIP;
+  }
+  *out = e;
+  return IEQ_OK;
+}
+
+int ensemble_step(Ensemble* e, long long nsteps, double* seconds, std::string& err) {
+  int rc = 0;
+  if (!e->tuned) {
+    rc = e->real_bytes == 8 ? ens_tune<double>(*e, err) : ens_tune<float>(*e, err);
+    if (rc) return IEQ_ERR_HIP;
+  }
+  if (nsteps % e->steps_per_graph) return err = "this integrator advances the ensemble in pairs of steps", IEQ_ERR_ARGS;
+  hipEvent_t t0, t1;
+  (void)hipEventCreate(&t0);
+  (void)hipEventCreate(&t1);
+  (void)hipEventRecord(t0, e->stream);
+  for (long long k = 0; k < nsteps; k += e->steps_per_graph)
+    if (hipGraphLaunch(e->exec, e->stream) != hipSuccess) return err = "graph launch failed", IEQ_ERR_HIP;
+  (void)hipEventRecord(t1, e->stream);
+  if (hipEventSynchronize(t1) != hipSuccess) return err = "ensemble step failed", IEQ_ERR_HIP;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, t0, t1);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (seconds) *seconds = ms * 1e-3;
+  return hipGetLastError() == hipSuccess ? IEQ_OK : (err = "ensemble kernel failed", IEQ_ERR_HIP);
+}
+
+void ensemble_config(const Ensemble* e, int* ty, int* zc, int* order) {
+  *ty = e->ty;
+  *zc = e->zc;
+  *order = e->order;
+}
+
+void ensemble_release(Ensemble* e) {
+  if (!e) return;
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  if (e->dargs) (void)hipFree(e->dargs);
+  if (e->dtiles) (void)hipFree(e->dtiles);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
 
 int fused_step(const Params& p, const ieq_program_info& info, hipStream_t s, FusedWork** work, const char* integrator,
                long long nsteps, int variable_u, void* const* args, int* used, std::string& err) {

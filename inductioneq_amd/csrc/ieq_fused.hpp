@@ -25,4 +25,14 @@ int fused_step(const Params& p, const ieq_program_info& info, hipStream_t s, Fus
 
 void fused_release(FusedWork* w);
 
+struct Ensemble;
+int ensemble_create(const Params& p, const ieq_program_info& info, const ieq_member* members, int n, const char* integrator,
+                    Ensemble** out, std::string& err);
+int ensemble_step(Ensemble* e, long long nsteps, double* seconds, std::string& err);
+void ensemble_config(const Ensemble* e, int* ty, int* zc, int* order);
+void ensemble_release(Ensemble* e);
+
+// Host copy of a -Laplacian table: lap 0 WS_D0, 1 NS_D0, 2 WS_LN; nullptr when absent.
+const double* host_lap_table(int opset, int lap, int* nbl, int* wl);
+
 }  // namespace ieq

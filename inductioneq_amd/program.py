@@ -157,5 +157,15 @@ class Program:
         self.run(name, list(fields) + [out], g=(self.info.num_nodes_pad, 1, 1), l=(self.info.w_size, 1, 1))
         return float(out.double().sum().item())
 
+    def clean_div_cg(self, b, phi=None, abs_threshold: float = 0.0, rel_tolerance: float = 0.0, max_iterations: int = 100):
+        """Divergence cleaning of b in place (CG on -Lap phi = div B, then B += grad phi)."""
+        phi = self.scalar() if phi is None else phi
+        res = _capi.CgResult()
+        rc = self.lib.ieq_clean_div_cg(self.handle, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(phi.data_ptr()),
+                                       float(abs_threshold), float(rel_tolerance), int(max_iterations), ctypes.byref(res))
+        _capi.check(rc, self.handle)
+        return {"divergence_norm_initial": res.divergence_norm_initial, "divergence_norm_final": res.divergence_norm_final,
+                "residual_norm": res.residual_norm, "iterations": res.iterations, "seconds": res.seconds, "phi": phi}
+
     def synchronize(self):
         _capi.check(self.lib.ieq_synchronize(self.handle), self.handle)

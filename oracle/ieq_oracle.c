@@ -842,6 +842,11 @@ int ieqo_run_kernel(const ieqo_cfg *cfg, const char *name, const uint32_t g[3], 
   k.ny = (uint32_t)cfg->ny;
   k.nz = (uint32_t)cfg->nz;
   set_threads(cfg);
+  if (cfg->periodic) { /* the single periodic wrap of utils.h:91-98 needs n >= stencil half-width */
+    int hw = (k.op->w - 1) / 2, lw = k.op->lap_w[cfg->laplace];
+    if (lw > 0 && (lw - 1) / 2 > hw) hw = (lw - 1) / 2;
+    if (cfg->nx < hw || cfg->ny < hw || cfg->nz < hw) return -3;
+  }
   for (int i = 0; i < NSTAGES; i++)
     if (!strcmp(STAGES[i].name, name)) {
       if (nargs != STAGES[i].nf + 4) return -2;

