@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--integrator", default="CarpenterKennedy2N54")
     ap.add_argument("--cpu-steps", type=int, default=15, help="RK steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["hall128", "ensemble64", "adjoint96"], default="hall128",
+                    help="hall128: one periodic Hall solve (the headline metric); ensemble64: 64 batched FP32 solves "
+                         "against the same solves run one by one in FP64")
+    ap.add_argument("--members", type=int, default=64, help="ensemble64: number of members")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -63,6 +67,12 @@ def main():
         print("bench.py: this build measures one GPU only (the workload is single-GPU; multi-GPU domain "
               "decomposition is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "ensemble64":
+        bench_ensemble(a)
+        return
+    if a.workload == "adjoint96":
+        bench_adjoint(a)
+        return
     import numpy as np
     import torch
 
@@ -133,6 +143,209 @@ def main():
                      "kernel": "one fused RK stage (k_emf + k_stage), %d B/cell algorithmic" % (4 * 3 * rb),
                      "us_per_stage": round(stage_s * 1e6, 2)},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_ensemble(a):
+    """64 periodic Hall solves (n in {32,48,64,96}, c_H in U(0.1, 2), B0 random modes with the member's seed,
+    SSPRK33, dt from the Hall CFL 0.4), 6th order + HO dissipation: FP32 ensemble (one persistent launch per
+    fused kernel per stage, hipGraph replay) vs the same solves one by one in FP64 (fused single-solve path)."""
+    import numpy as np
+    import torch
+
+    from inductioneq_amd.ensemble import Ensemble, MemberSpec
+    from inductioneq_amd.problems import hall_dt, random_modes_field
+    from inductioneq_amd.program import Program
+    from inductioneq_amd.settings import ProgramConfig
+
+    rng = np.random.default_rng(2024)
+    sizes = rng.choice([32, 48, 64, 96], size=a.members)
+    c_h = rng.uniform(0.1, 2.0, size=a.members)
+    L = 2 * math.pi
+    integ = "SSPRK33"
+    steps = a.steps if a.steps != 40 else 200
+
+    def cfg_for(n, real, dt=1e-3):
+        return ProgramConfig(real=real, opset="6thOrder", nx=n, ny=n, nz=n, xmin=0, xmax=L, ymin=0, ymax=L, zmin=0,
+                             zmax=L, periodic=True, hall=True, dissipation="HIGH_ORDER", testcase="uniform_flow",
+                             w_size=256, dt=dt)
+
+    B0s, specs = [], []
+    for i, (n, c) in enumerate(zip(sizes, c_h)):
+        n = int(n)
+        B0 = random_modes_field(n, seed=i)
+        h = L / n
+        dt = hall_dt(B0, (h, h, h), 6, c_hall=float(c), cfl=0.4)
+        B0s.append(B0)
+        specs.append(MemberSpec(n, n, n, h, h, h, dt, rho=1.0 / float(c)))
+    ncell = int(sum(int(n) ** 3 for n in sizes))
+    print("bench: ensemble of %d members, %d cells, %d steps" % (a.members, ncell, steps), file=sys.stderr, flush=True)
+
+    prog32 = Program(cfg_for(32, "float"))
+    ens = Ensemble(prog32, specs, integ)
+
+    def load():  # B0 is node-major (x fastest): the member layout of b1 directly
+        for i, B0 in enumerate(B0s):
+            ens.fields[i][0][:, : B0.shape[0]] = torch.as_tensor(B0.T, dtype=torch.float32, device="cuda")
+
+    load()
+    ens.step(1)  # tunes the launch shape (fields restored), then one step
+    load()
+    torch.cuda.synchronize()
+    sec32 = ens.step(steps)
+    cfg_e = ens.config()
+    print("bench: fp32 ensemble %.3f s %s" % (sec32, cfg_e), file=sys.stderr, flush=True)
+
+    sec64, err = 0.0, 0.0
+    for i, B0 in enumerate(B0s):
+        n = specs[i].nx
+        prog = Program(cfg_for(n, "double", specs[i].dt))
+        prog.use_current_stream()
+        b1 = prog.vector()
+        b1[:, : n ** 3] = torch.as_tensor(B0.T, dtype=torch.float64, device="cuda")
+        args = [b1, prog.vector(), prog.vector(), prog.vector(), prog.vector(), prog.scalar(specs[i].rho),
+                prog.time_buffer()]
+        torch.cuda.synchronize()
+        sec64 += prog.step(integ, steps, args, timed=True)
+        ref = b1[:, : n ** 3].double()
+        got = ens.fields[i][0][:, : n ** 3].double()
+        err = max(err, float(((got - ref).abs().max() / ref.abs().max()).item()))
+        del prog, b1, args
+    stages = 3 * steps
+    value = ncell * stages / sec32 / 1e6
+    rb = 4
+    alg = 64 // 2 * ncell  # SSPRK33 average 8/3 vector fields per stage: 32 B/cell FP32
+    out = {
+        "metric": "Mcell-updates/s per RK stage",
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 1,
+        "ms_per_step": round(sec32 * 1e3 / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "ensemble_%d_hall_periodic_6th_HOD_SSPRK33" % a.members, "members": a.members,
+                   "cells": ncell, "sizes": [int(v) for v in sizes], "launch": cfg_e},
+        "roofline": {"bound": "hbm", "achieved": round(alg / (sec32 / stages) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg / (sec32 / stages) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "one batched RK stage (k_emf_batched + k_stage_batched), 32 B/cell algorithmic"},
+        "per_instance_fp64": {"seconds": round(sec64, 4), "value": round(ncell * stages / sec64 / 1e6, 3)},
+        "speedup_vs_per_instance_fp64": round(sec64 / sec32, 3),
+        "max_rel_err_vs_fp64": err,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(a):
+    """Gradient of 0.5 h^3 |B(T) - B_target|^2 after 200 SSPRK33 steps of the periodic Hall problem at 96^3
+    (6th order + HO dissipation, u = (0.2, -0.1, 0.15), c_H = 1), w.r.t. B0 (FP32 state), u and c_H; B_target
+    is the same solve from a different seed.  Binomial checkpointing with 15 snapshots.  Reports the gradient
+    time over the plain forward time, and central finite differences (eps = 1e-4, FP64) of the FP64 gradient."""
+    import numpy as np
+    import torch
+
+    from inductioneq_amd.adjoint import AdjointSolver
+    from inductioneq_amd.problems import hall_dt, random_modes_field
+    from inductioneq_amd.program import Program
+    from inductioneq_amd.settings import ProgramConfig
+
+    n = 96 if a.n == 128 else a.n
+    steps = a.steps if a.steps != 40 else 200
+    snaps = 15
+    L = 2 * math.pi
+    u, ch = [0.2, -0.1, 0.15], 1.0
+    B0 = random_modes_field(n, seed=1)
+    Bt = random_modes_field(n, seed=2)
+    h = L / n
+    dt = hall_dt(B0, (h, h, h), 6, c_hall=ch, cfl=0.4)
+    res = {}
+    for real in ("float", "double"):
+        cfg = ProgramConfig(real=real, opset="6thOrder", nx=n, ny=n, nz=n, xmin=0, xmax=L, ymin=0, ymax=L, zmin=0, zmax=L,
+                            periodic=True, hall=True, dissipation="HIGH_ORDER", testcase="uniform_flow", w_size=256,
+                            dt=dt)
+        prog = Program(cfg)
+        prog.use_current_stream()
+        N3 = n ** 3
+        b0, tgt0 = prog.vector(), prog.vector()
+        b0[:, :N3] = torch.as_tensor(B0.T, dtype=prog.torch_dtype)
+        tgt0[:, :N3] = torch.as_tensor(Bt.T, dtype=prog.torch_dtype)
+        solver = AdjointSolver(prog, steps, snaps)
+        tgt = solver.forward(tgt0, u, ch)  # B_target: the different-seed forward run
+        # plain forward (fused ieq_step, same kernels) for the cost ratio
+        b1 = b0.clone()
+        uf = prog.vector()
+        for k in range(3):
+            uf[k, :N3] = u[k]
+        args = [b1, prog.vector(), prog.vector(), prog.vector(), uf, prog.scalar(1.0 / ch), prog.time_buffer()]
+        prog.step("SSPRK33", 2, args)  # warm-up
+        b1.copy_(b0)
+        t_plain = prog.step("SSPRK33", steps, args, timed=True)
+        bT = solver.forward(b0, u, ch)
+        t_fwd = solver.last_forward_seconds
+        lam = (bT - tgt) * (h ** 3)
+        g, gc, gu = solver.backward(lam)
+        t_bwd = solver.last_backward_seconds
+        nf, na = solver.counts()
+        d = (bT - tgt)[:, :N3].double()
+        loss = 0.5 * h ** 3 * float((d * d).sum())
+        res[real] = {"t_plain": t_plain, "t_fwd": t_fwd, "t_bwd": t_bwd, "grad": g[:, :N3].double().cpu(), "gc": gc,
+                     "gu": gu, "loss": loss, "counts": (nf, na)}
+        print("bench: adjoint %s plain %.3f s, forward %.3f s, backward %.3f s, steps fwd %d adj %d" %
+              (real, t_plain, t_fwd, t_bwd, nf, na), file=sys.stderr, flush=True)
+        if real == "double":  # central finite differences of the FP64 gradient
+            eps = 1e-4
+
+            def L_of(bb, uu, cc):
+                d = (solver.forward(bb, uu, cc) - tgt)[:, :N3].double()
+                return 0.5 * h ** 3 * float((d * d).sum())
+
+            v = prog.vector()
+            v[:, :N3] = torch.as_tensor(np.random.default_rng(7).standard_normal((3, N3)), dtype=torch.float64)
+            fd = {"b0": (L_of(b0 + eps * v, u, ch) - L_of(b0 - eps * v, u, ch)) / (2 * eps)}
+            ad = {"b0": float((g[:, :N3] * v[:, :N3]).sum())}
+            for k in range(3):
+                up, um = list(u), list(u)
+                up[k] += eps
+                um[k] -= eps
+                fd["u%d" % k] = (L_of(b0, up, ch) - L_of(b0, um, ch)) / (2 * eps)
+                ad["u%d" % k] = gu[k]
+            fd["c_hall"] = (L_of(b0, u, ch + eps) - L_of(b0, u, ch - eps)) / (2 * eps)
+            ad["c_hall"] = gc
+            res[real]["fd"] = {k: {"fd": fd[k], "adjoint": ad[k], "rel_err": abs(fd[k] - ad[k]) / abs(fd[k])}
+                               for k in fd}
+        del solver, prog
+    r32, r64 = res["float"], res["double"]
+    gerr = float((r32["grad"] - r64["grad"]).abs().max() / r64["grad"].abs().max())
+    ratio = (r32["t_fwd"] + r32["t_bwd"]) / r32["t_plain"]
+    stages = 3 * steps
+    out = {
+        "metric": "adjoint gradient cost / forward solve",
+        "value": round(ratio, 3),
+        "unit": "x forward",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 2,
+        "ms_per_step": round((r32["t_fwd"] + r32["t_bwd"]) * 1e3 / steps, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "adjoint_hall_periodic_%d^3_6th_HOD_SSPRK33" % n, "grid": [n, n, n], "steps": steps,
+                   "snapshots": snaps, "u": u, "c_hall": ch, "dt": dt, "seeds": [1, 2]},
+        "forward_Mcell_per_s": round(n ** 3 * stages / r32["t_plain"] / 1e6, 3),
+        "timings_s": {k: {"plain_forward": round(v["t_plain"], 5), "checkpointed_forward": round(v["t_fwd"], 5),
+                          "backward": round(v["t_bwd"], 5), "forward_steps": v["counts"][0],
+                          "adjoint_steps": v["counts"][1]} for k, v in res.items()},
+        "fp32_vs_fp64_grad_rel_err": gerr,
+        "fd_check_fp64": r64["fd"],
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

@@ -183,6 +183,28 @@ int ieq_ensemble_config(const ieq_ensemble *ens, int *tile_rows, int *z_chunk, i
 
 void ieq_ensemble_release(ieq_ensemble *ens);
 
+/* Discrete adjoint of a periodic SSPRK33 solve of the program (fused stage arithmetic,
+ * uniform velocity u and Hall coefficient c_hall = 1/rho given at run time, HO or no
+ * dissipation).  No counterpart in the reference; it differentiates the time loop of
+ * compute_numerical_solution.m:91-108 run with the SSPRK33 kernels
+ * (kernel_time_integrator.cl:224-303).  `snapshots` vector fields are kept for
+ * optimal binomial checkpointing of the nsteps steps (stage inputs are recomputed from the
+ * step start); snapshots >= nsteps stores every step. */
+typedef struct ieq_adjoint ieq_adjoint;
+int ieq_adjoint_create(ieq_adjoint **adj, ieq_program *prog, long long nsteps, int snapshots);
+
+/* bT = B(T) after nsteps steps from b0 (vector fields of the program); stores the snapshots. */
+int ieq_adjoint_forward(ieq_adjoint *adj, const void *b0, void *bT, double c_hall, const double u[3], double *kernel_seconds);
+
+/* Given lambda_T = dL/dB(T): grad_b0 = dL/dB0, grads = {dL/dc_hall, dL/du_x, dL/du_y, dL/du_z}
+ * (FP64, accumulated in a fixed order).  Uses the state of the last ieq_adjoint_forward. */
+int ieq_adjoint_backward(ieq_adjoint *adj, const void *lambda_T, void *grad_b0, double grads[4], double *kernel_seconds);
+
+/* Forward steps taken (forward sweep plus recomputation) and adjoint steps of the last calls. */
+int ieq_adjoint_counts(const ieq_adjoint *adj, long long *forward_steps, long long *adjoint_steps);
+
+void ieq_adjoint_release(ieq_adjoint *adj);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,11 +23,12 @@ EXPORTS = [
     "ieq_synchronize", "ieq_alloc", "ieq_free", "ieq_vector_field_bytes", "ieq_scalar_field_bytes",
     "ieq_upload_real4", "ieq_download_real4", "ieq_upload_real", "ieq_download_real", "ieq_run", "ieq_step",
     "ieq_clean_div_cg", "ieq_member_stride", "ieq_ensemble_create", "ieq_ensemble_step", "ieq_ensemble_config",
-    "ieq_ensemble_release",
+    "ieq_ensemble_release", "ieq_adjoint_create", "ieq_adjoint_forward", "ieq_adjoint_backward", "ieq_adjoint_counts",
+    "ieq_adjoint_release",
 ]
 
 _NOT_INT = ("ieq_program_release", "ieq_last_error", "ieq_vector_field_bytes", "ieq_scalar_field_bytes",
-            "ieq_member_stride", "ieq_ensemble_release")
+            "ieq_member_stride", "ieq_ensemble_release", "ieq_adjoint_release")
 
 
 class CgResult(ctypes.Structure):
@@ -102,6 +103,14 @@ def load() -> ctypes.CDLL:
     lib.ieq_ensemble_config.argtypes = [P, ip, ip, ip]
     lib.ieq_ensemble_release.argtypes = [P]
     lib.ieq_ensemble_release.restype = None
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.ieq_adjoint_create.argtypes = [ctypes.POINTER(P), P, ctypes.c_longlong, ctypes.c_int]
+    lib.ieq_adjoint_forward.argtypes = [P, P, P, ctypes.c_double, dp, dp]
+    lib.ieq_adjoint_backward.argtypes = [P, P, P, dp, dp]
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    lib.ieq_adjoint_counts.argtypes = [P, llp, llp]
+    lib.ieq_adjoint_release.argtypes = [P]
+    lib.ieq_adjoint_release.restype = None
     for fn in EXPORTS:
         if fn not in _NOT_INT:
             getattr(lib, fn).restype = ctypes.c_int

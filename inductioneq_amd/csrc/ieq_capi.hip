@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/inductioneq.h"
+#include "ieq_adjoint.hpp"
 #include "ieq_cg.hpp"
 #include "ieq_device.hpp"
 #include "ieq_fused.hpp"
@@ -779,6 +780,43 @@ void ieq_ensemble_release(ieq_ensemble* ens) {
   if (!ens) return;
   ieq::ensemble_release(ens->e);
   delete ens;
+}
+
+struct ieq_adjoint {
+  ieq_program* prog;
+  ieq::Adjoint* a;
+};
+
+int ieq_adjoint_create(ieq_adjoint** adj, ieq_program* prog, long long nsteps, int snapshots) {
+  if (!adj || !prog) return IEQ_ERR_ARGS;
+  *adj = nullptr;
+  ieq::Adjoint* a = nullptr;
+  int rc = ieq::adjoint_create(prog->p, prog->info, nsteps, snapshots, &a, prog->err);
+  if (rc) return rc;
+  *adj = new ieq_adjoint{prog, a};
+  return IEQ_OK;
+}
+
+int ieq_adjoint_forward(ieq_adjoint* adj, const void* b0, void* bT, double c_hall, const double u[3], double* kernel_seconds) {
+  if (!adj || !b0 || !bT) return IEQ_ERR_ARGS;
+  return ieq::adjoint_forward(adj->a, adj->prog->stream, b0, bT, c_hall, u, kernel_seconds, adj->prog->err);
+}
+
+int ieq_adjoint_backward(ieq_adjoint* adj, const void* lambda_T, void* grad_b0, double grads[4], double* kernel_seconds) {
+  if (!adj || !lambda_T || !grad_b0 || !grads) return IEQ_ERR_ARGS;
+  return ieq::adjoint_backward(adj->a, adj->prog->stream, lambda_T, grad_b0, grads, kernel_seconds, adj->prog->err);
+}
+
+int ieq_adjoint_counts(const ieq_adjoint* adj, long long* forward_steps, long long* adjoint_steps) {
+  if (!adj || !forward_steps || !adjoint_steps) return IEQ_ERR_ARGS;
+  ieq::adjoint_counts(adj->a, forward_steps, adjoint_steps);
+  return IEQ_OK;
+}
+
+void ieq_adjoint_release(ieq_adjoint* adj) {
+  if (!adj) return;
+  ieq::adjoint_release(adj->a);
+  delete adj;
 }
 
 }  // extern "C"

@@ -19,6 +19,7 @@
 #include "ieq_device.hpp"
 #include "ieq_fused.hpp"
 #include "ieq_generic.hpp"
+#include "ieq_stencil.hpp"
 
 namespace ieq {
 
@@ -106,143 +107,6 @@ int schedule(const char* integrator, int hall, int variable_u, std::vector<std::
 }
 
 namespace {
-
-// ------------------------------------------------------------------------------------
-template <typename R> struct StencilCoef {
-  R dp[4];  // central first-derivative row: coefficient at offset +k (k = 1..r)
-  R dm[4];  // ... at offset -k
-  R w[4];   // dissipation weights at offsets +-k (k = 0..r), HO_DISSIPATION_FACTOR folded in
-};
-
-template <typename R> struct FArgs {
-  int nx, ny, nz, zc;
-  long long S;
-  StencilCoef<R> c;
-  R inv_h[3];  // (R)(1/dx_d): Hall_volume scaling
-  R two_h[3];  // (R)(2/dx_d): convec_volume scaling
-  R h[3];      // (R)dx_d: derivative divisor of curl
-  R rho;       // uniform density (J = curl B / rho)
-  R u[3];      // uniform velocity
-  int conv;    // velocity non-zero: evaluate the convection term
-  int src_zero;
-  const R* Y;  // field differentiated (B of the stage)
-  const R* E;  // EMF (k_stage, Hall)
-  R* Eout;     // EMF output (k_emf)
-  int mode;    // 0: out = cP*P + cQ*Q + cD*RHS;  1 (2N): b2 = cP*b2 + cD*RHS, out = Y + cQ*b2
-  R* out;
-  const R* P;
-  const R* Q;
-  R* b2;
-  R cP, cQ, cD;
-};
-
-// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8) and
-// each XCD has its own L2.  Giving XCD k the contiguous tile range
-// [start_k, start_k + count_k) keeps the halo rows a tile shares with its y- and z-neighbours
-// inside one L2 while they are being read.  Bijective for any nb.
-constexpr int kXcds = 8;
-__device__ inline int xcd_tile(int b, int nb) {
-  const int q = nb / kXcds, r = nb % kXcds, k = b % kXcds;
-  return k * q + min(k, r) + b / kXcds;
-}
-
-// Linear tile index (x fastest, then y, then z-chunk) -> tile origin.
-template <int TY> __device__ inline void tile_origin(int t, int nx, int ny, int zc, int& x0, int& y0, int& z0) {
-  const int ntx = (nx + 63) / 64, nty = (ny + TY - 1) / TY;
-  x0 = (t % ntx) * 64;
-  t /= ntx;
-  y0 = (t % nty) * TY;
-  z0 = (t / nty) * zc;
-}
-
-__device__ inline int wrap(int v, int n) {
-  v %= n;
-  return v < 0 ? v + n : v;
-}
-
-template <typename R, int RR> __device__ inline R deriv(const StencilCoef<R>& c, const R* v) {
-  // v[RR + k] holds the value at offset k
-  R acc = R(0);
-#pragma unroll
-  for (int k = RR; k >= 1; --k) acc = acc + c.dm[k] * v[RR - k];
-#pragma unroll
-  for (int k = 1; k <= RR; ++k) acc = acc + c.dp[k] * v[RR + k];
-  return acc;
-}
-
-template <typename R, int RR> __device__ inline R hodsum(const StencilCoef<R>& c, const R* v) {
-  R acc = c.w[0] * v[RR];
-#pragma unroll
-  for (int k = 1; k <= RR; ++k) acc = acc + c.w[k] * (v[RR - k] + v[RR + k]);
-  return acc;
-}
-
-// Barrier over LDS traffic only: the LDS stores of this plane are visible after it,
-// while global prefetch loads of the next plane stay in flight across it.
-__device__ inline void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// Per-thread staging of one xy-plane of an NC-component field into an LDS tile with a
-// halo of RR cells.  The thread owns its column (x0+tx, y0+ty) and, at the tile edges,
-// up to one x-halo cell and up to two y-halo cells (all coordinates wrap periodically).
-template <typename R, int RR, int TY, int NC> struct PlaneStage {
-  static constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR;
-  // With TY >= 2*RR a thread stages at most one y-halo cell (top rows or bottom rows).
-  static constexpr bool ONE_Y = TY >= 2 * RR;
-  bool hx, ht, hb;
-  int lcx, lry;                // LDS column of the x-halo cell; LDS row of the y-halo cell (ONE_Y)
-  long long gx, gt, gb;        // in-plane global indices of the halo cells
-  R vx[NC], vt[NC], vb[ONE_Y ? 1 : NC];  // prefetched halo values of the next plane to stage
-
-  __device__ void init(int tx, int ty, int x0, int y0, int nx, int ny, long long NX) {
-    const int xw = wrap(x0 + tx, nx), yw = wrap(y0 + ty, ny);
-    hx = tx < RR || tx >= 64 - RR;
-    ht = ty < RR;
-    hb = ty >= TY - RR;
-    lcx = tx < RR ? tx : tx + 2 * RR;
-    gx = (tx < RR ? wrap(x0 - RR + tx, nx) : wrap(x0 + tx + RR, nx)) + NX * yw;
-    gt = xw + NX * wrap(y0 - RR + ty, ny);
-    gb = xw + NX * wrap(y0 + ty + RR, ny);
-    if (ONE_Y) {
-      lry = ht ? ty : ty + 2 * RR;
-      if (hb) gt = gb;
-      ht = ht || hb;
-    }
-  }
-  __device__ void load(const R* f, long long S, long long pz) {
-    if (hx)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) vx[c] = f[c * S + gx + pz];
-    if (ht)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) vt[c] = f[c * S + gt + pz];
-    if constexpr (!ONE_Y) {
-      if (hb)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) vb[c] = f[c * S + gb + pz];
-    }
-  }
-  __device__ void store(R (*sm)[LY][LX], int tx, int ty) {
-    if (hx)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) sm[c][ty + RR][lcx] = vx[c];
-    if constexpr (ONE_Y) {
-      if (ht)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) sm[c][lry][tx + RR] = vt[c];
-    } else {
-      if (ht)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) sm[c][ty][tx + RR] = vt[c];
-      if (hb)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) sm[c][ty + 2 * RR][tx + RR] = vb[c];
-    }
-  }
-};
 
 // EMF kernel: E = (curl_D B / rho) x B.  Software pipelined along z: plane z is
 // computed from LDS buffer (z & 1) while the loads of plane z+1 are in flight.
@@ -343,7 +207,7 @@ template <typename R, int RR, int TY, bool HALL> struct StageLds {
   R sE[2][NE][LYE][LXE];
 };
 
-template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV>
+template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ = false>
 __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, int z0, StageLds<R, RR, TY, HALL>& L) {
   constexpr int Q = 2 * RR + 1;
   auto& sB = L.sB;
@@ -407,11 +271,16 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
     }
     const long long ic = col + NXY * z;
     R pv[3], qv[3];  // pointwise RK operands, loaded before the barrier
+    R gv[ADJ ? 3 : 1];
     if (active) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         pv[c] = a.mode == 0 ? (a.P ? a.P[c * S + ic] : R(0)) : a.b2[c * S + ic];
         qv[c] = (a.mode == 0 && a.Q) ? a.Q[c * S + ic] : R(0);
+      }
+      if constexpr (ADJ) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gv[c] = a.G ? a.G[c * S + ic] : R(0);
       }
     }
     lds_barrier();
@@ -437,7 +306,10 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
             const R der = deriv<R, RR>(a.c, v);
             const R f = a.two_h[d] * R(0.5);
             rhs[c] = rhs[c] - f * (a.u[d] * der);
-            if (c == d && a.src_zero) {
+            if constexpr (ADJ) {
+              if (a.src_adj) rhs[d] = rhs[d] + f * (a.u[c] * der);
+            }
+            if (c == d && a.src_zero && !(ADJ && a.src_adj)) {
 #pragma unroll
               for (int e = 0; e < 3; ++e) rhs[e] = rhs[e] + f * (a.u[e] * der);
             }
@@ -478,10 +350,15 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
           rhs[c] = rhs[c] + h;
         }
       }
+      if constexpr (ADJ) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rhs[c] = rhs[c] - gv[c];
+      }
       if (a.mode == 0) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           R r = a.cD * rhs[c];
+          if constexpr (ADJ) r = a.cY * qb[c][RR] + r;
           if (a.Q) r = a.cQ * qv[c] + r;
           if (a.P) r = a.cP * pv[c] + r;
           a.out[c * S + ic] = r;
@@ -507,12 +384,12 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
   }
 }
 
-template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV>
+template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ = false>
 __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(IEQ_STAGE_WAVES, 8))) void k_stage(FArgs<R> a) {
   __shared__ StageLds<R, RR, TY, HALL> L;
   int x0, y0, z0;
   tile_origin<TY>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
-  stage_tile<R, RR, TY, HALL, HOD, CONV>(a, x0, y0, z0, L);
+  stage_tile<R, RR, TY, HALL, HOD, CONV, ADJ>(a, x0, y0, z0, L);
 }
 
 template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV>
@@ -579,7 +456,12 @@ template <typename R, int RR, int TY>
 void launch_stage_ty(const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
   const int grid = ((a.nx + 63) / 64) * ((a.ny + TY - 1) / TY) * ((a.nz + a.zc - 1) / a.zc);
   dim3 blk(64, TY);
-  if (a.conv) {
+  if (a.G || a.src_adj || a.cY != R(0)) {  // transposed stage (adjoint sweeps)
+    if (hall && hod) k_stage<R, RR, TY, true, true, true, true><<<grid, blk, 0, s>>>(a);
+    else if (hall) k_stage<R, RR, TY, true, false, true, true><<<grid, blk, 0, s>>>(a);
+    else if (hod) k_stage<R, RR, TY, false, true, true, true><<<grid, blk, 0, s>>>(a);
+    else k_stage<R, RR, TY, false, false, true, true><<<grid, blk, 0, s>>>(a);
+  } else if (a.conv) {
     if (hall && hod) k_stage<R, RR, TY, true, true, true><<<grid, blk, 0, s>>>(a);
     else if (hall) k_stage<R, RR, TY, true, false, true><<<grid, blk, 0, s>>>(a);
     else if (hod) k_stage<R, RR, TY, false, true, true><<<grid, blk, 0, s>>>(a);
@@ -628,6 +510,16 @@ struct HostOp {
 };
 
 }  // namespace
+
+template <typename R> void launch_emf(int rr, int ty, const FArgs<R>& a, hipStream_t s) { emf<R>(rr, ty, a, s); }
+template <typename R> void launch_stage(int rr, int ty, const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
+  stage<R>(rr, ty, a, hall, hod, s);
+}
+template void launch_emf<double>(int, int, const FArgs<double>&, hipStream_t);
+template void launch_emf<float>(int, int, const FArgs<float>&, hipStream_t);
+template void launch_stage<double>(int, int, const FArgs<double>&, bool, bool, hipStream_t);
+template void launch_stage<float>(int, int, const FArgs<float>&, bool, bool, hipStream_t);
+
 }  // namespace ieq
 
 // host-side copies of the tables for coefficient extraction
@@ -671,8 +563,6 @@ const double* host_lap_table(int opset, int lap, int* nbl, int* wl) {
   return tabs[opset][lap].t;
 }
 
-namespace {
-
 // Stencil half-width and coefficients of the central rows; returns -1 when the
 // central row has non-zero entries beyond the dissipation half-width.
 template <typename R> int central_coefs(int ops, double ho_factor, StencilCoef<R>& c) {
@@ -696,12 +586,6 @@ template <typename R> int central_coefs(int ops, double ho_factor, StencilCoef<R
   for (int k = 0; k <= rr; ++k) c.w[k] = (R)(ho_factor * w[rr + k]);
   return rr;
 }
-
-struct StageSpec {
-  int kind;  // 0: K_STAGE, 1: 2R update, 2: SSPRK104_06, 3: EMF, 4: 2N fused (a+b)
-  int out, P, Q, Y;
-  double cP, cQ, cD;  // K_STAGE: cD multiplies DT; 2R: (a, b); 2N: A (cP), B (cQ)
-};
 
 // Fused plan of one step (same arithmetic as the schedule, time kernels dropped).
 bool fused_plan(const std::string& I, int hall, std::vector<StageSpec>& v, int& nf) {
@@ -771,6 +655,11 @@ bool fused_plan(const std::string& I, int hall, std::vector<StageSpec>& v, int& 
   }
   return false;
 }
+
+template int central_coefs<double>(int, double, StencilCoef<double>&);
+template int central_coefs<float>(int, double, StencilCoef<float>&);
+
+namespace {
 
 template <typename R>
 int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, FusedWork** work, const char* integrator,

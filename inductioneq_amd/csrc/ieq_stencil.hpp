@@ -1,0 +1,171 @@
+// Stencil machinery shared by the fused stage kernels, ensembles and the adjoint:
+// central-row coefficients, per-launch stage arguments, tile mapping, z-marching
+// helpers, and the fused forward launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace ieq {
+
+// ------------------------------------------------------------------------------------
+template <typename R> struct StencilCoef {
+  R dp[4];  // central first-derivative row: coefficient at offset +k (k = 1..r)
+  R dm[4];  // ... at offset -k
+  R w[4];   // dissipation weights at offsets +-k (k = 0..r), HO_DISSIPATION_FACTOR folded in
+};
+
+template <typename R> struct FArgs {
+  int nx, ny, nz, zc;
+  long long S;
+  StencilCoef<R> c;
+  R inv_h[3];  // (R)(1/dx_d): Hall_volume scaling
+  R two_h[3];  // (R)(2/dx_d): convec_volume scaling
+  R h[3];      // (R)dx_d: derivative divisor of curl
+  R rho;       // uniform density (J = curl B / rho)
+  R u[3];      // uniform velocity
+  int conv;    // velocity non-zero: evaluate the convection term
+  int src_zero;
+  const R* Y;  // field differentiated (B of the stage)
+  const R* E;  // EMF (k_stage, Hall)
+  R* Eout;     // EMF output (k_emf)
+  int mode;    // 0: out = cP*P + cQ*Q + cD*RHS;  1 (2N): b2 = cP*b2 + cD*RHS, out = Y + cQ*b2
+  R* out;
+  const R* P;
+  const R* Q;
+  R* b2;
+  R cP, cQ, cD;
+  // adjoint stage only (stage kernels instantiated with ADJ = true):
+  const R* G;   // pointwise term subtracted from the RHS (null: none)
+  R cY;         // out += cY * Y at the node
+  int src_adj;  // source term in transposed form: rhs_d += (1/dx_d) sum_c u_c D_d Y_c
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8) and
+// each XCD has its own L2.  Giving XCD k the contiguous tile range
+// [start_k, start_k + count_k) keeps the halo rows a tile shares with its y- and z-neighbours
+// inside one L2 while they are being read.  Bijective for any nb.
+constexpr int kXcds = 8;
+__device__ inline int xcd_tile(int b, int nb) {
+  const int q = nb / kXcds, r = nb % kXcds, k = b % kXcds;
+  return k * q + min(k, r) + b / kXcds;
+}
+
+// Linear tile index (x fastest, then y, then z-chunk) -> tile origin.
+template <int TY> __device__ inline void tile_origin(int t, int nx, int ny, int zc, int& x0, int& y0, int& z0) {
+  const int ntx = (nx + 63) / 64, nty = (ny + TY - 1) / TY;
+  x0 = (t % ntx) * 64;
+  t /= ntx;
+  y0 = (t % nty) * TY;
+  z0 = (t / nty) * zc;
+}
+
+__device__ inline int wrap(int v, int n) {
+  v %= n;
+  return v < 0 ? v + n : v;
+}
+
+template <typename R, int RR> __device__ inline R deriv(const StencilCoef<R>& c, const R* v) {
+  // v[RR + k] holds the value at offset k
+  R acc = R(0);
+#pragma unroll
+  for (int k = RR; k >= 1; --k) acc = acc + c.dm[k] * v[RR - k];
+#pragma unroll
+  for (int k = 1; k <= RR; ++k) acc = acc + c.dp[k] * v[RR + k];
+  return acc;
+}
+
+template <typename R, int RR> __device__ inline R hodsum(const StencilCoef<R>& c, const R* v) {
+  R acc = c.w[0] * v[RR];
+#pragma unroll
+  for (int k = 1; k <= RR; ++k) acc = acc + c.w[k] * (v[RR - k] + v[RR + k]);
+  return acc;
+}
+
+// Barrier over LDS traffic only: the LDS stores of this plane are visible after it,
+// while global prefetch loads of the next plane stay in flight across it.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Per-thread staging of one xy-plane of an NC-component field into an LDS tile with a
+// halo of RR cells.  The thread owns its column (x0+tx, y0+ty) and, at the tile edges,
+// up to one x-halo cell and up to two y-halo cells (all coordinates wrap periodically).
+template <typename R, int RR, int TY, int NC> struct PlaneStage {
+  static constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR;
+  // With TY >= 2*RR a thread stages at most one y-halo cell (top rows or bottom rows).
+  static constexpr bool ONE_Y = TY >= 2 * RR;
+  bool hx, ht, hb;
+  int lcx, lry;                // LDS column of the x-halo cell; LDS row of the y-halo cell (ONE_Y)
+  long long gx, gt, gb;        // in-plane global indices of the halo cells
+  R vx[NC], vt[NC], vb[ONE_Y ? 1 : NC];  // prefetched halo values of the next plane to stage
+
+  __device__ void init(int tx, int ty, int x0, int y0, int nx, int ny, long long NX) {
+    const int xw = wrap(x0 + tx, nx), yw = wrap(y0 + ty, ny);
+    hx = tx < RR || tx >= 64 - RR;
+    ht = ty < RR;
+    hb = ty >= TY - RR;
+    lcx = tx < RR ? tx : tx + 2 * RR;
+    gx = (tx < RR ? wrap(x0 - RR + tx, nx) : wrap(x0 + tx + RR, nx)) + NX * yw;
+    gt = xw + NX * wrap(y0 - RR + ty, ny);
+    gb = xw + NX * wrap(y0 + ty + RR, ny);
+    if (ONE_Y) {
+      lry = ht ? ty : ty + 2 * RR;
+      if (hb) gt = gb;
+      ht = ht || hb;
+    }
+  }
+  __device__ void load(const R* f, long long S, long long pz) {
+    if (hx)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) vx[c] = f[c * S + gx + pz];
+    if (ht)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) vt[c] = f[c * S + gt + pz];
+    if constexpr (!ONE_Y) {
+      if (hb)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) vb[c] = f[c * S + gb + pz];
+    }
+  }
+  __device__ void store(R (*sm)[LY][LX], int tx, int ty) {
+    if (hx)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sm[c][ty + RR][lcx] = vx[c];
+    if constexpr (ONE_Y) {
+      if (ht)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sm[c][lry][tx + RR] = vt[c];
+    } else {
+      if (ht)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sm[c][ty][tx + RR] = vt[c];
+      if (hb)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sm[c][ty + 2 * RR][tx + RR] = vb[c];
+    }
+  }
+};
+
+// Stencil half-width RR of the central rows and their coefficients (derivative and
+// HO dissipation, HO_DISSIPATION_FACTOR folded in); -1 when the central row is wider.
+template <typename R> int central_coefs(int ops, double ho_factor, StencilCoef<R>& c);
+
+struct StageSpec {
+  int kind;  // 0: K_STAGE, 1: 2R update, 2: SSPRK104_06, 3: EMF, 4: 2N fused (a+b)
+  int out, P, Q, Y;
+  double cP, cQ, cD;  // K_STAGE: cD multiplies DT; 2R: (a, b); 2N: A (cP), B (cQ)
+};
+
+// Fused plan of one step (same arithmetic as the reference schedule, time kernels dropped).
+bool fused_plan(const std::string& I, int hall, std::vector<StageSpec>& v, int& nf);
+
+// Fused forward launches (TY in {4, 8}, RR in {1, 2, 3}).
+template <typename R> void launch_emf(int rr, int ty, const FArgs<R>& a, hipStream_t s);
+template <typename R> void launch_stage(int rr, int ty, const FArgs<R>& a, bool hall, bool hod, hipStream_t s);
+
+}  // namespace ieq

@@ -255,3 +255,91 @@ def test_cg_stops_on_the_absolute_threshold_like_the_reference():
     b2 = _aos(cfg, _rand(cfg, 1))
     res2 = clean_div_cg(cfg, o, b2, thr, res["iterations"] - 1)
     assert res2["residual_norm"] > thr  # one iteration fewer does not reach it
+
+
+def _m_weights(cfg, key, periodic):
+    t = T[key]
+    d = [np.diag(norm_matrix(t["minv"], n, periodic)) * h for n, h in ((cfg.nx, cfg.dx), (cfg.ny, cfg.dy), (cfg.nz, cfg.dz))]
+    return np.einsum("k,j,i->kji", d[2], d[1], d[0]).reshape(-1)
+
+
+def _hall_energy_rate(ops, periodic, seed):
+    n = (14, 13, 15)
+    lo, hi = (0.0, 2 * math.pi) if periodic else (-1.0, 1.0)
+    cfg = ProgramConfig(opset=ops, nx=n[0], ny=n[1], nz=n[2], xmin=lo, xmax=hi, ymin=lo, ymax=hi, zmin=lo, zmax=hi,
+                        periodic=periodic, w_size=16, testcase="hall_travelling_wave", hall=True)
+    o = Oracle(cfg)
+    B = _rand(cfg, seed)
+    hb = _aos(cfg, B)
+    args = [hb, o.vec(), o.vec(), o.vec(), o.vec(), np.ones(cfg.num_nodes_pad), o.time()]
+    o.run("calc_curlB_rho_1_3args", args)
+    rhs = o.rhs(hb, args[3], o.vec())[: cfg.num_nodes, :3]
+    w = _m_weights(cfg, OPS_NAME[ops], periodic)[:, None]
+    return (w * B * rhs).sum(), (w * np.abs(B * rhs)).sum()
+
+
+@pytest.mark.parametrize("ops", ["2ndOrder", "4thOrder", "6thOrder", "4thOrderExtended"])
+def test_hall_term_conserves_energy_on_periodic_grids(ops):
+    """B^T M RHS = -(curl B).(J x B) = 0 for the Hall term alone: skew-symmetric periodic SBP operators."""
+    for seed in range(3):
+        rate, scale = _hall_energy_rate(ops, True, seed)
+        assert abs(rate) <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("ops", ["2ndOrder", "4thOrder", "6thOrder", "4thOrderExtended"])
+def test_hall_outflow_sat_dissipates_energy(ops):
+    """With walls, Hall_volume + the Hall outflow SAT (Hall_surface) give a non-positive M-energy rate."""
+    for seed in range(3):
+        rate, scale = _hall_energy_rate(ops, False, seed)
+        assert rate < -1e-3 * scale
+
+
+def _diss_only(ops, n, periodic, diss, B, U, **kw):
+    """Dissipation term alone: RHS with it minus RHS without it (same convection code path)."""
+    out = []
+    for d in (diss, None):
+        cfg = _cfg(ops, n, periodic, testcase="uniform_flow", dissipation=d, **kw)
+        o = Oracle(cfg)
+        out.append(o.rhs(_aos(cfg, B), o.vec(), _aos(cfg, U))[: cfg.num_nodes, :3])
+    return out[0] - out[1], np.abs(out[1]).max()
+
+
+def test_first_order_dissipation_is_rusanov_viscosity_for_constant_velocity():
+    """2nd order, periodic, constant u: sum_d |u_c|/(2 dx_d) (B(+e_d) - 2B + B(-e_d)) per component (Lax-Friedrichs)."""
+    n = (12, 11, 13)
+    cfg = _cfg("2ndOrder", n, True)
+    B = _rand(cfg, 8)
+    u = np.array([0.7, -1.3, 0.4])
+    got, conv = _diss_only("2ndOrder", n, True, "FIRST_ORDER", B, np.tile(u, (cfg.num_nodes, 1)))
+    F = B.reshape(n[2], n[1], n[0], 3)
+    want = np.zeros_like(F)
+    for ax, h in ((2, cfg.dx), (1, cfg.dy), (0, cfg.dz)):
+        want += np.abs(u) / (2 * h) * (np.roll(F, 1, ax) - 2 * F + np.roll(F, -1, ax))
+    assert np.abs(got - want.reshape(-1, 3)).max() <= 1e-12 * max(conv, np.abs(want).max())
+
+
+@pytest.mark.parametrize("ops", ["2ndOrder", "4thOrder", "6thOrder"])
+def test_first_order_dissipation_dissipates_energy_with_variable_velocity(ops):
+    """Weights |D_i| max(|u_k|,|u_n|) are symmetric in (n, k) on periodic grids: B.FO(B) <= 0, = 0 for constant B."""
+    n = (12, 11, 13)
+    cfg = _cfg(ops, n, True)
+    B, U = _rand(cfg, 9), _rand(cfg, 10)
+    got, conv = _diss_only(ops, n, True, "FIRST_ORDER", B, U)
+    assert (B * got).sum() < -1e-3 * np.abs(B * got).sum()
+    got_c, conv_c = _diss_only(ops, n, True, "FIRST_ORDER", np.ones_like(B) * [0.3, -2.0, 1.1], U)
+    assert np.abs(got_c).max() <= 1e-12 * max(conv_c, 1.0)
+
+
+@pytest.mark.parametrize("ops", ["2ndOrder", "4thOrder", "6thOrder"])
+def test_adaptive_dissipation_limits(ops):
+    """kappa = 0 everywhere (jumps below the MP2MIN*CMIN threshold) gives the high-order dissipation;
+    kappa = 1 everywhere (jumps above MP2MAX*CMAX) gives the first-order dissipation."""
+    n = (12, 11, 13)
+    cfg = _cfg(ops, n, True)
+    B, U = _rand(cfg, 11), _rand(cfg, 12)
+    ad0, c0 = _diss_only(ops, n, True, "ADAPTIVE", B, U, ho_factor=0.7, mp2min=1e30, mp2max=2e30, cmin=1.0, cmax=1.0)
+    ho, c1 = _diss_only(ops, n, True, "HIGH_ORDER", B, U, ho_factor=0.7)
+    assert np.abs(ad0 - ho).max() <= 1e-12 * max(c0, np.abs(ho).max())
+    ad1, c2 = _diss_only(ops, n, True, "ADAPTIVE", B, U, ho_factor=0.7, mp2min=0.0, mp2max=1e-300, cmin=1.0, cmax=1.0)
+    fo, c3 = _diss_only(ops, n, True, "FIRST_ORDER", B, U)
+    assert np.abs(ad1 - fo).max() <= 1e-12 * max(c2, np.abs(fo).max())
