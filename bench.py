@@ -36,7 +36,8 @@ def parse():
     ap.add_argument("--integrator", default="CarpenterKennedy2N54")
     ap.add_argument("--cpu-steps", type=int, default=15, help="RK steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["hall128", "ensemble64", "adjoint96"], default="hall128",
+    ap.add_argument("--workload", choices=["hall128", "ensemble64", "adjoint96", "rotation40", "cg192"],
+                    default="hall128",
                     help="hall128: one periodic Hall solve (the headline metric); ensemble64: 64 batched FP32 solves "
                          "against the same solves run one by one in FP64")
     ap.add_argument("--members", type=int, default=64, help="ensemble64: number of members")
@@ -72,6 +73,12 @@ def main():
         return
     if a.workload == "adjoint96":
         bench_adjoint(a)
+        return
+    if a.workload == "rotation40":
+        bench_rotation(a)
+        return
+    if a.workload == "cg192":
+        bench_cg(a)
         return
     import numpy as np
     import torch
@@ -345,6 +352,138 @@ def bench_adjoint(a):
                           "adjoint_steps": v["counts"][1]} for k, v in res.items()},
         "fp32_vs_fp64_grad_rel_err": gerr,
         "fd_check_fp64": r64["fd"],
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_rotation(a):
+    """Linear induction equation on [-1,1]^3 at 40^3, 4th-order SBP with SAT boundaries, u = (-y, x, 0),
+    B0 = curl(0, 0, exp(-20((x-1/2)^2 + y^2 + z^2))), SSPRK33 at CFL 0.5 to t = pi/2 (reference-schedule
+    kernels, FP64).  Reports throughput, the error against the rotated analytic field, and the full-run
+    difference to the CPU oracle (which is also the CPU baseline)."""
+    import torch
+
+    from inductioneq_amd.harness import HipExecutor, compute_numerical_solution, initialize
+    from inductioneq_amd.settings import ProgramConfig
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    n = 40 if a.n == 128 else a.n
+    T = math.pi / 2
+    cfg = ProgramConfig(real="double", opset="4thOrder", nx=n, ny=n, nz=n, xmin=-1, xmax=1, ymin=-1, ymax=1, zmin=-1,
+                        zmax=1, periodic=False, hall=False, dissipation=None, testcase="gaussian_rotation_3D",
+                        form_uibj="PRODUCT", form_source="CENTRAL", form_ujbi="CENTRAL", w_size=64)
+    init = initialize(cfg, HipExecutor, 0.5, T)
+    cfg.dt = init["dt"]
+    ex = HipExecutor(cfg)
+    ex.prog.use_current_stream()
+    r = compute_numerical_solution(cfg, ex, "SSPRK33", init["b"], init["u"], init["rho"], 2, 2 * cfg.dt, fused=True)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    res = compute_numerical_solution(cfg, ex, "SSPRK33", init["b"], init["u"], init["rho"], init["num_steps"], T,
+                                     fused=True)
+    e1.record()
+    torch.cuda.synchronize()
+    stages = 3 * init["num_steps"]
+    sec = res["runtime"]
+    value = n ** 3 * stages / sec / 1e6
+    cpu, parity = None, None
+    if not a.no_cpu_baseline:
+        from pyoracle import Oracle
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        orc = Oracle(cfg, nthreads=threads)
+        t0 = time.perf_counter()
+        ro = compute_numerical_solution(cfg, orc, "SSPRK33", init["b"], init["u"], init["rho"], init["num_steps"], T)
+        dt_cpu = time.perf_counter() - t0
+        cpu = {"value": round(n ** 3 * stages / ro["runtime"] / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
+               "kind": "port", "sample": "the whole run (%d steps, %d stages), %.2f s" % (init["num_steps"], stages,
+                                                                                         dt_cpu)}
+        import numpy as np
+
+        d = np.abs(res["field_b"] - ro["field_b"]).max() / np.abs(ro["field_b"]).max()
+        parity = {"max_rel_diff_final_field": float(d), "oracle_rel_err": ro["rel_err"]}
+    out = {
+        "metric": "Mcell-updates/s per RK stage",
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": init["num_steps"],
+        "warmup": 2,
+        "ms_per_step": round(sec * 1e3 / init["num_steps"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "rotation_%d^3_4th_SAT_SSPRK33" % n, "dt": cfg.dt, "final_time": T},
+        "rel_err_vs_analytic": res["rel_err"],
+        "divergence_norm": res["divergence_norm"],
+        "oracle_parity": parity,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_cg(a):
+    """Divergence cleaning at 192^3 on [-1,1]^3 (4th-order SBP, WS_D0 Laplacian): B = the rotation40 field plus
+    grad psi (Fourier |k| <= 6, seed 3), CG to relative residual 1e-10; FP32 stencils with FP64 dot products
+    against all-FP64.  Reports iterations, time per iteration and the divergence norm before and after."""
+    import numpy as np
+    import torch
+
+    from inductioneq_amd.problems import gaussian_rotation_field, gradient_potential
+    from inductioneq_amd.program import Program
+    from inductioneq_amd.settings import ProgramConfig
+
+    n = 192 if a.n == 128 else a.n
+    B = gaussian_rotation_field(n) + gradient_potential(n, seed=3)
+    res = {}
+    for real in ("double", "float"):
+        cfg = ProgramConfig(real=real, opset="4thOrder", nx=n, ny=n, nz=n, xmin=-1, xmax=1, ymin=-1, ymax=1, zmin=-1,
+                            zmax=1, periodic=False, testcase="rotation_3D", laplace="WIDE_STENCIL_DIRICHLET", w_size=256)
+        prog = Program(cfg)
+        prog.use_current_stream()
+        b = prog.vector()
+        b[:, : n ** 3] = torch.as_tensor(B.T, dtype=prog.torch_dtype)
+        warm = prog.vector()
+        warm.copy_(b)
+        prog.clean_div_cg(warm, rel_tolerance=1e-10, max_iterations=20)
+        r = prog.clean_div_cg(b, rel_tolerance=1e-10, max_iterations=20000)
+        r["b"] = b[:, : n ** 3].double().cpu()
+        res[real] = r
+        print("bench: cg %s %d iterations, %.3f s, div %.3e -> %.3e" % (real, r["iterations"], r["seconds"],
+              r["divergence_norm_initial"], r["divergence_norm_final"]), file=sys.stderr, flush=True)
+        del prog
+    r64, r32 = res["double"], res["float"]
+    it_s = r32["seconds"] / max(r32["iterations"], 1)
+    cells = (n - 2) ** 3
+    alg = 40 * cells  # FP32 fields: K1 reads u, r, writes u, q; K2 reads phi, u, r, q, writes phi, r (10 scalars)
+    diff = float((r32["b"] - r64["b"]).abs().max() / r64["b"].abs().max())
+    out = {
+        "metric": "CG iterations/s",
+        "value": round(1.0 / it_s, 1),
+        "unit": "iterations/s",
+        "n_gpus": 1,
+        "steps": r32["iterations"],
+        "warmup": 20,
+        "ms_per_step": round(it_s * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 stencil / f64 dots",
+        "data": "synthetic",
+        "config": {"workload": "divclean_%d^3_4th_WS_D0_rel1e-10" % n, "seed": 3},
+        "fp32": {k: r32[k] for k in ("iterations", "seconds", "divergence_norm_initial", "divergence_norm_final",
+                                     "residual_norm")},
+        "fp64": {k: r64[k] for k in ("iterations", "seconds", "divergence_norm_initial", "divergence_norm_final",
+                                     "residual_norm")},
+        "fp32_vs_fp64_cleaned_field_max_rel_diff": diff,
+        "roofline": {"bound": "hbm", "achieved": round(alg / it_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / it_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "one CG iteration (k_cg_dir + k_cg_lap + k_cg_update), 40 B/cell algorithmic FP32"},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -135,10 +135,12 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
 #pragma unroll
     for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = q[c][RR];
     hs.store(sB[buf], tx, ty);
-    {  // prefetch plane z+RR+2 of the column and the halo of plane z+1
-      const long long id = col + NXY * wrap(z + RR + 2, a.nz);
+    {  // prefetch plane z+RR+2 of the column (used from step z+2 on) and the halo of plane z+1
+      if (z + 2 < zend) {
+        const long long id = col + NXY * wrap(z + RR + 2, a.nz);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) q[c][Q + 1] = B[c * S + id];
+        for (int c = 0; c < 3; ++c) q[c][Q + 1] = B[c * S + id];
+      }
       if (z + 1 < zend) hs.load(B, S, NXY * (z + 1));
     }
     lds_barrier();
@@ -253,13 +255,15 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
       sE[buf][2][ty + RR][tx + RR] = ez;
       hse.store(sE[buf], tx, ty);
     }
-    {  // prefetch: column plane z+RR+2, halo plane z+1
-      const long long id = col + NXY * wrap(z + RR + 2, a.nz);
+    {  // prefetch: column plane z+RR+2 (used from step z+2 on), halo plane z+1
+      if (z + 2 < zend) {
+        const long long id = col + NXY * wrap(z + RR + 2, a.nz);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) qb[c][Q + 1] = B[c * S + id];
-      if constexpr (HALL) {
-        qe[0][Q + 1] = E[id];
-        qe[1][Q + 1] = E[S + id];
+        for (int c = 0; c < 3; ++c) qb[c][Q + 1] = B[c * S + id];
+        if constexpr (HALL) {
+          qe[0][Q + 1] = E[id];
+          qe[1][Q + 1] = E[S + id];
+        }
       }
       if (z + 1 < zend) {
         hsb.load(B, S, NXY * (z + 1));
