@@ -110,7 +110,7 @@ namespace {
 
 // EMF kernel: E = (curl_D B / rho) x B.  Software pipelined along z: plane z is
 // computed from LDS buffer (z & 1) while the loads of plane z+1 are in flight.
-template <typename R, int RR, int TY>
+template <typename R, int RR, int TY, int NBUF = 2>
 __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int z0, R (*sB)[3][TY + 2 * RR][64 + 2 * RR]) {
   constexpr int Q = 2 * RR + 1;
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -131,7 +131,12 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
   }
   hs.load(B, S, NXY * z0);
   for (int z = z0; z < zend; ++z) {
-    const int buf = (z - z0) & 1;
+    // NBUF == 2: plane z goes to the buffer plane z-1 did not use; NBUF == 1: wait until every
+    // thread has read plane z-1 before overwriting it
+    const int buf = NBUF == 2 ? (z - z0) & 1 : 0;
+    if constexpr (NBUF == 1) {
+      if (z > z0) lds_barrier();
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = q[c][RR];
     hs.store(sB[buf], tx, ty);
@@ -175,12 +180,12 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
   }
 }
 
-template <typename R, int RR, int TY>
+template <typename R, int RR, int TY, int NBUF>
 __global__ __launch_bounds__(64 * TY) void k_emf(FArgs<R> a) {
-  __shared__ R sB[2][3][TY + 2 * RR][64 + 2 * RR];
+  __shared__ R sB[NBUF][3][TY + 2 * RR][64 + 2 * RR];
   int x0, y0, z0;
   tile_origin<TY>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
-  emf_tile<R, RR, TY>(a, x0, y0, z0, sB);
+  emf_tile<R, RR, TY, NBUF>(a, x0, y0, z0, sB);
 }
 
 // Persistent batched form: blocks stride over a tile list covering every member.
@@ -202,15 +207,15 @@ __global__ __launch_bounds__(64 * TY) void k_emf_batched(const FArgs<R>* __restr
 #ifndef IEQ_STAGE_WAVES
 #define IEQ_STAGE_WAVES 2
 #endif
-template <typename R, int RR, int TY, bool HALL> struct StageLds {
+template <typename R, int RR, int TY, bool HALL, int NBUF = 2> struct StageLds {
   static constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR;
   static constexpr int NE = HALL ? 3 : 1, LYE = HALL ? LY : 1, LXE = HALL ? LX : 1;
-  R sB[2][3][LY][LX];
-  R sE[2][NE][LYE][LXE];
+  R sB[NBUF][3][LY][LX];
+  R sE[NBUF][NE][LYE][LXE];
 };
 
-template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ = false>
-__device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, int z0, StageLds<R, RR, TY, HALL>& L) {
+template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ = false, int NBUF = 2>
+__device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, int z0, StageLds<R, RR, TY, HALL, NBUF>& L) {
   constexpr int Q = 2 * RR + 1;
   auto& sB = L.sB;
   auto& sE = L.sE;
@@ -245,7 +250,10 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
     ez = E[2 * S + col + NXY * z0];
   }
   for (int z = z0; z < zend; ++z) {
-    const int buf = (z - z0) & 1;
+    const int buf = NBUF == 2 ? (z - z0) & 1 : 0;
+    if constexpr (NBUF == 1) {
+      if (z > z0) lds_barrier();
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = qb[c][RR];
     hsb.store(sB[buf], tx, ty);
@@ -388,12 +396,12 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
   }
 }
 
-template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ = false>
+template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ, int NBUF>
 __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(IEQ_STAGE_WAVES, 8))) void k_stage(FArgs<R> a) {
-  __shared__ StageLds<R, RR, TY, HALL> L;
+  __shared__ StageLds<R, RR, TY, HALL, NBUF> L;
   int x0, y0, z0;
   tile_origin<TY>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
-  stage_tile<R, RR, TY, HALL, HOD, CONV, ADJ>(a, x0, y0, z0, L);
+  stage_tile<R, RR, TY, HALL, HOD, CONV, ADJ, NBUF>(a, x0, y0, z0, L);
 }
 
 template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV>
@@ -450,44 +458,53 @@ template <typename R> __global__ void k_advance_time(R* time, R dt, long long ns
   }
 }
 
-template <typename R, int RR, int TY>
+template <typename R, int RR, int TY, int NBUF>
 void launch_emf_ty(const FArgs<R>& a, hipStream_t s) {
   const int grid = ((a.nx + 63) / 64) * ((a.ny + TY - 1) / TY) * ((a.nz + a.zc - 1) / a.zc);
-  k_emf<R, RR, TY><<<grid, dim3(64, TY), 0, s>>>(a);
+  k_emf<R, RR, TY, NBUF><<<grid, dim3(64, TY), 0, s>>>(a);
+}
+
+template <typename R, int RR, int TY, bool ADJ, int NBUF>
+void launch_stage_hdc(const FArgs<R>& a, bool hall, bool hod, bool conv, hipStream_t s) {
+  const int grid = ((a.nx + 63) / 64) * ((a.ny + TY - 1) / TY) * ((a.nz + a.zc - 1) / a.zc);
+  dim3 blk(64, TY);
+  if (conv) {
+    if (hall && hod) k_stage<R, RR, TY, true, true, true, ADJ, NBUF><<<grid, blk, 0, s>>>(a);
+    else if (hall) k_stage<R, RR, TY, true, false, true, ADJ, NBUF><<<grid, blk, 0, s>>>(a);
+    else if (hod) k_stage<R, RR, TY, false, true, true, ADJ, NBUF><<<grid, blk, 0, s>>>(a);
+    else k_stage<R, RR, TY, false, false, true, ADJ, NBUF><<<grid, blk, 0, s>>>(a);
+  } else {
+    if (hall && hod) k_stage<R, RR, TY, true, true, false, ADJ, NBUF><<<grid, blk, 0, s>>>(a);
+    else if (hall) k_stage<R, RR, TY, true, false, false, ADJ, NBUF><<<grid, blk, 0, s>>>(a);
+    else if (hod) k_stage<R, RR, TY, false, true, false, ADJ, NBUF><<<grid, blk, 0, s>>>(a);
+    else k_stage<R, RR, TY, false, false, false, ADJ, NBUF><<<grid, blk, 0, s>>>(a);
+  }
 }
 
 template <typename R, int RR, int TY>
 void launch_stage_ty(const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
-  const int grid = ((a.nx + 63) / 64) * ((a.ny + TY - 1) / TY) * ((a.nz + a.zc - 1) / a.zc);
-  dim3 blk(64, TY);
-  if (a.G || a.src_adj || a.cY != R(0)) {  // transposed stage (adjoint sweeps)
-    if (hall && hod) k_stage<R, RR, TY, true, true, true, true><<<grid, blk, 0, s>>>(a);
-    else if (hall) k_stage<R, RR, TY, true, false, true, true><<<grid, blk, 0, s>>>(a);
-    else if (hod) k_stage<R, RR, TY, false, true, true, true><<<grid, blk, 0, s>>>(a);
-    else k_stage<R, RR, TY, false, false, true, true><<<grid, blk, 0, s>>>(a);
-  } else if (a.conv) {
-    if (hall && hod) k_stage<R, RR, TY, true, true, true><<<grid, blk, 0, s>>>(a);
-    else if (hall) k_stage<R, RR, TY, true, false, true><<<grid, blk, 0, s>>>(a);
-    else if (hod) k_stage<R, RR, TY, false, true, true><<<grid, blk, 0, s>>>(a);
-    else k_stage<R, RR, TY, false, false, true><<<grid, blk, 0, s>>>(a);
-  } else {
-    if (hall && hod) k_stage<R, RR, TY, true, true, false><<<grid, blk, 0, s>>>(a);
-    else if (hall) k_stage<R, RR, TY, true, false, false><<<grid, blk, 0, s>>>(a);
-    else if (hod) k_stage<R, RR, TY, false, true, false><<<grid, blk, 0, s>>>(a);
-    else k_stage<R, RR, TY, false, false, false><<<grid, blk, 0, s>>>(a);
-  }
+  if (a.G || a.src_adj || a.cY != R(0))  // transposed stage (adjoint sweeps): convection always evaluated
+    launch_stage_hdc<R, RR, TY, true, 2>(a, hall, hod, true, s);
+  else if (a.nbuf == 1)
+    launch_stage_hdc<R, RR, TY, false, 1>(a, hall, hod, a.conv, s);
+  else
+    launch_stage_hdc<R, RR, TY, false, 2>(a, hall, hod, a.conv, s);
 }
 
-template <typename R> void emf(int rr, int ty, const FArgs<R>& a, hipStream_t s) {
-  if (ty == 8) {
-    if (rr == 1) launch_emf_ty<R, 1, 8>(a, s);
-    else if (rr == 2) launch_emf_ty<R, 2, 8>(a, s);
-    else launch_emf_ty<R, 3, 8>(a, s);
+template <typename R, int TY> void emf_t(int rr, const FArgs<R>& a, hipStream_t s) {
+  if (a.nbuf == 1) {
+    if (rr == 1) launch_emf_ty<R, 1, TY, 1>(a, s);
+    else if (rr == 2) launch_emf_ty<R, 2, TY, 1>(a, s);
+    else launch_emf_ty<R, 3, TY, 1>(a, s);
   } else {
-    if (rr == 1) launch_emf_ty<R, 1, 4>(a, s);
-    else if (rr == 2) launch_emf_ty<R, 2, 4>(a, s);
-    else launch_emf_ty<R, 3, 4>(a, s);
+    if (rr == 1) launch_emf_ty<R, 1, TY, 2>(a, s);
+    else if (rr == 2) launch_emf_ty<R, 2, TY, 2>(a, s);
+    else launch_emf_ty<R, 3, TY, 2>(a, s);
   }
+}
+template <typename R> void emf(int rr, int ty, const FArgs<R>& a, hipStream_t s) {
+  if (ty == 8) emf_t<R, 8>(rr, a, s);
+  else emf_t<R, 4>(rr, a, s);
 }
 template <typename R> void stage(int rr, int ty, const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
   if (ty == 8) {
@@ -720,6 +737,7 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
   a.nz = p.nz;
   a.zc = env_int("IEQ_FUSED_ZC", 8);
   if (a.zc < 1) a.zc = 1;
+  a.nbuf = env_int("IEQ_FUSED_NBUF", 2) == 1 ? 1 : 2;
   const int ty = env_int("IEQ_FUSED_TY", 8) == 4 ? 4 : 8;
   a.S = info.stride;
   a.c = coef;

@@ -19,6 +19,7 @@ template <typename R> struct StencilCoef {
 
 template <typename R> struct FArgs {
   int nx, ny, nz, zc;
+  int nbuf;  // LDS plane buffers of the single-grid kernels: 2 (pipelined) or 1 (half the LDS)
   long long S;
   StencilCoef<R> c;
   R inv_h[3];  // (R)(1/dx_d): Hall_volume scaling
