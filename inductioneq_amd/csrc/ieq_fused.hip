@@ -140,13 +140,14 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
 #pragma unroll
     for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = q[c][RR];
     hs.store(sB[buf], tx, ty);
-    {  // prefetch plane z+RR+2 of the column (used from step z+2 on) and the halo of plane z+1
-      if (z + 2 < zend) {
-        const long long id = col + NXY * wrap(z + RR + 2, a.nz);
+    // Loads are issued in the order they are consumed: the halo of plane z+1 (next step),
+    // then column plane z+RR+2 (two steps on).  The memory counter retires in issue order
+    // and also counts this step's stores, so the next step waits only for the halo.
+    if (z + 1 < zend) hs.load(B, S, NXY * (z + 1));
+    if (z + 2 < zend) {
+      const long long id = col + NXY * wrap(z + RR + 2, a.nz);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) q[c][Q + 1] = B[c * S + id];
-      }
-      if (z + 1 < zend) hs.load(B, S, NXY * (z + 1));
+      for (int c = 0; c < 3; ++c) q[c][Q + 1] = B[c * S + id];
     }
     lds_barrier();
     if (active) {
@@ -214,6 +215,21 @@ template <typename R, int RR, int TY, bool HALL, int NBUF = 2> struct StageLds {
   R sE[NBUF][NE][LYE][LXE];
 };
 
+// Pointwise operands of the RK update at node ic: P (or b2 in 2N mode), Q, and the adjoint's G.
+template <bool ADJ, typename R>
+__device__ __forceinline__ void load_point(const FArgs<R>& a, long long ic, R* pv, R* qv, R* gv) {
+  const long long S = a.S;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    pv[c] = a.mode == 0 ? (a.P ? a.P[c * S + ic] : R(0)) : a.b2[c * S + ic];
+    qv[c] = (a.mode == 0 && a.Q) ? a.Q[c * S + ic] : R(0);
+  }
+  if constexpr (ADJ) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gv[c] = a.G ? a.G[c * S + ic] : R(0);
+  }
+}
+
 template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ = false, int NBUF = 2>
 __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, int z0, StageLds<R, RR, TY, HALL, NBUF>& L) {
   constexpr int Q = 2 * RR + 1;
@@ -249,6 +265,9 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
     hse.load(E, S, NXY * z0);
     ez = E[2 * S + col + NXY * z0];
   }
+  // pointwise RK operands of the current plane (pv, qv, gv) and of the next (pn, qn, gn)
+  R pv[3], qv[3], gv[ADJ ? 3 : 1], pn[3], qn[3], gn[ADJ ? 3 : 1];
+  if (active) load_point<ADJ>(a, col + NXY * z0, pv, qv, gv);
   for (int z = z0; z < zend; ++z) {
     const int buf = NBUF == 2 ? (z - z0) & 1 : 0;
     if constexpr (NBUF == 1) {
@@ -263,38 +282,27 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
       sE[buf][2][ty + RR][tx + RR] = ez;
       hse.store(sE[buf], tx, ty);
     }
-    {  // prefetch: column plane z+RR+2 (used from step z+2 on), halo plane z+1
-      if (z + 2 < zend) {
-        const long long id = col + NXY * wrap(z + RR + 2, a.nz);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) qb[c][Q + 1] = B[c * S + id];
-        if constexpr (HALL) {
-          qe[0][Q + 1] = E[id];
-          qe[1][Q + 1] = E[S + id];
-        }
+    // Loads in the order they are consumed (the memory counter retires in issue order and
+    // also counts stores): halo of plane z+1 (next step), pointwise operands of plane z+1
+    // (end of next step), column plane z+RR+2 (two steps on).
+    if (z + 1 < zend) {
+      hsb.load(B, S, NXY * (z + 1));
+      if constexpr (HALL) {
+        hse.load(E, S, NXY * (z + 1));
+        ez = E[2 * S + col + NXY * (z + 1)];
       }
-      if (z + 1 < zend) {
-        hsb.load(B, S, NXY * (z + 1));
-        if constexpr (HALL) {
-          hse.load(E, S, NXY * (z + 1));
-          ez = E[2 * S + col + NXY * (z + 1)];
-        }
+      if (active) load_point<ADJ>(a, col + NXY * (z + 1), pn, qn, gn);
+    }
+    if (z + 2 < zend) {
+      const long long id = col + NXY * wrap(z + RR + 2, a.nz);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) qb[c][Q + 1] = B[c * S + id];
+      if constexpr (HALL) {
+        qe[0][Q + 1] = E[id];
+        qe[1][Q + 1] = E[S + id];
       }
     }
     const long long ic = col + NXY * z;
-    R pv[3], qv[3];  // pointwise RK operands, loaded before the barrier
-    R gv[ADJ ? 3 : 1];
-    if (active) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        pv[c] = a.mode == 0 ? (a.P ? a.P[c * S + ic] : R(0)) : a.b2[c * S + ic];
-        qv[c] = (a.mode == 0 && a.Q) ? a.Q[c * S + ic] : R(0);
-      }
-      if constexpr (ADJ) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) gv[c] = a.G ? a.G[c * S + ic] : R(0);
-      }
-    }
     lds_barrier();
     if (active) {
       R rhs[3] = {R(0), R(0), R(0)};
@@ -383,6 +391,15 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
           a.out[c * S + ic] = qb[c][RR] + a.cQ * b2;
         }
       }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pv[c] = pn[c];
+      qv[c] = qn[c];
+    }
+    if constexpr (ADJ) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) gv[c] = gn[c];
     }
 #pragma unroll
     for (int j = 0; j < Q + 1; ++j) {
