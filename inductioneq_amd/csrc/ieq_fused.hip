@@ -108,8 +108,14 @@ int schedule(const char* integrator, int hall, int variable_u, std::vector<std::
 
 namespace {
 
-// EMF kernel: E = (curl_D B / rho) x B.  Software pipelined along z: plane z is
-// computed from LDS buffer (z & 1) while the loads of plane z+1 are in flight.
+// EMF kernel: E = (curl_D B / rho) x B.
+//
+// Lead-aligned z-march: iteration `it` stages plane p = it + RR (the newest plane of the
+// z-queue) into LDS together with its x/y halo and evaluates the x/y-derivative terms of
+// J for that plane; the z-derivative terms of plane `it` come from the register queue and
+// complete J(it).  The column value and the halo of a plane are therefore read in the same
+// iteration, which is also when the neighbouring tiles (dispatched together to the same
+// XCD) read those rows as their own column values, so the halo rows are L2 hits.
 template <typename R, int RR, int TY, int NBUF = 2>
 __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int z0, R (*sB)[3][TY + 2 * RR][64 + 2 * RR]) {
   constexpr int Q = 2 * RR + 1;
@@ -122,62 +128,75 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
   PlaneStage<R, RR, TY, 3> hs;
   hs.init(tx, ty, x0, y0, a.nx, a.ny, NX);
 
-  R q[3][Q + 2];  // B at planes z-RR .. z+RR+2 of the own column
+  R q[3][Q];       // own column: planes it-RR .. it+RR
+  R ql[3];         // landing register: plane it+RR+1
+  R jr[3][RR + 1];  // x/y parts of curl_D B for planes it .. it+RR: (d_y B_z, -d_x B_z, d_x B_y - d_y B_x)
+  const int it0 = z0 - RR;
 #pragma unroll
-  for (int j = 0; j < Q + 1; ++j) {
-    const long long id = col + NXY * wrap(z0 - RR + j, a.nz);
+  for (int j = RR; j < Q; ++j) {  // planes z0-RR .. z0 (older slots are never read)
+    const long long id = col + NXY * wrap(it0 - RR + j, a.nz);
 #pragma unroll
     for (int c = 0; c < 3; ++c) q[c][j] = B[c * S + id];
   }
   hs.load(B, S, NXY * z0);
-  for (int z = z0; z < zend; ++z) {
-    // NBUF == 2: plane z goes to the buffer plane z-1 did not use; NBUF == 1: wait until every
-    // thread has read plane z-1 before overwriting it
-    const int buf = NBUF == 2 ? (z - z0) & 1 : 0;
+  for (int it = it0; it < zend; ++it) {
+    const int p = it + RR;  // plane staged in this iteration
+    const int buf = NBUF == 2 ? (it - it0) & 1 : 0;
     if constexpr (NBUF == 1) {
-      if (z > z0) lds_barrier();
+      if (it > it0) lds_barrier();
     }
+    if (p < zend) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = q[c][RR];
-    hs.store(sB[buf], tx, ty);
-    // Loads are issued in the order they are consumed: the halo of plane z+1 (next step),
-    // then column plane z+RR+2 (two steps on).  The memory counter retires in issue order
-    // and also counts this step's stores, so the next step waits only for the halo.
-    if (z + 1 < zend) hs.load(B, S, NXY * (z + 1));
-    if (z + 2 < zend) {
-      const long long id = col + NXY * wrap(z + RR + 2, a.nz);
+      for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = q[c][Q - 1];
+      hs.store(sB[buf], tx, ty);
+    }
+    // next iteration's loads, halo first (the memory counter retires in issue order)
+    if (p + 1 < zend) hs.load(B, S, NXY * (p + 1));
+    if (it + 1 < zend) {
+      const long long id = col + NXY * wrap(p + 1, a.nz);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) q[c][Q + 1] = B[c * S + id];
+      for (int c = 0; c < 3; ++c) ql[c] = B[c * S + id];
     }
     lds_barrier();
     if (active) {
-      R vx[Q], vy[Q];
+      if (p < zend) {
+        R vx[Q], vy[Q];
 #pragma unroll
-      for (int k = 0; k < Q; ++k) vx[k] = sB[buf][1][ty + RR][tx + k];
-      const R dxBy = deriv<R, RR>(a.c, vx) / a.h[0];
+        for (int k = 0; k < Q; ++k) vx[k] = sB[buf][1][ty + RR][tx + k];
+        const R dxBy = deriv<R, RR>(a.c, vx) / a.h[0];
 #pragma unroll
-      for (int k = 0; k < Q; ++k) vx[k] = sB[buf][2][ty + RR][tx + k];
-      const R dxBz = deriv<R, RR>(a.c, vx) / a.h[0];
+        for (int k = 0; k < Q; ++k) vx[k] = sB[buf][2][ty + RR][tx + k];
+        const R dxBz = deriv<R, RR>(a.c, vx) / a.h[0];
 #pragma unroll
-      for (int k = 0; k < Q; ++k) vy[k] = sB[buf][0][ty + k][tx + RR];
-      const R dyBx = deriv<R, RR>(a.c, vy) / a.h[1];
+        for (int k = 0; k < Q; ++k) vy[k] = sB[buf][0][ty + k][tx + RR];
+        const R dyBx = deriv<R, RR>(a.c, vy) / a.h[1];
 #pragma unroll
-      for (int k = 0; k < Q; ++k) vy[k] = sB[buf][2][ty + k][tx + RR];
-      const R dyBz = deriv<R, RR>(a.c, vy) / a.h[1];
-      const R dzBx = deriv<R, RR>(a.c, q[0]) / a.h[2];
-      const R dzBy = deriv<R, RR>(a.c, q[1]) / a.h[2];
-      // J = curl B / rho  (SBP_operator.cl:126-135, kernel_time_integrator.cl:147)
-      const R Jx = (dyBz - dzBy) / a.rho, Jy = (dzBx - dxBz) / a.rho, Jz = (dxBy - dyBx) / a.rho;
-      const R Bx = q[0][RR], By = q[1][RR], Bz = q[2][RR];
-      const long long ic = col + NXY * z;
-      a.Eout[ic] = Jy * Bz - Jz * By;
-      a.Eout[S + ic] = Jz * Bx - Jx * Bz;
-      a.Eout[2 * S + ic] = Jx * By - Jy * Bx;
+        for (int k = 0; k < Q; ++k) vy[k] = sB[buf][2][ty + k][tx + RR];
+        const R dyBz = deriv<R, RR>(a.c, vy) / a.h[1];
+        jr[0][RR] = dyBz;
+        jr[1][RR] = -dxBz;
+        jr[2][RR] = dxBy - dyBx;
+      }
+      if (it >= z0) {
+        const R dzBx = deriv<R, RR>(a.c, q[0]) / a.h[2];
+        const R dzBy = deriv<R, RR>(a.c, q[1]) / a.h[2];
+        // J = curl B / rho  (SBP_operator.cl:126-135, kernel_time_integrator.cl:147)
+        const R Jx = (jr[0][0] - dzBy) / a.rho, Jy = (dzBx + jr[1][0]) / a.rho, Jz = jr[2][0] / a.rho;
+        const R Bx = q[0][RR], By = q[1][RR], Bz = q[2][RR];
+        const long long ic = col + NXY * it;
+        a.Eout[ic] = Jy * Bz - Jz * By;
+        a.Eout[S + ic] = Jz * Bx - Jx * Bz;
+        a.Eout[2 * S + ic] = Jx * By - Jy * Bx;
+      }
     }
 #pragma unroll
-    for (int j = 0; j < Q + 1; ++j)
+    for (int c = 0; c < 3; ++c) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) q[c][j] = q[c][j + 1];
+      for (int j = 0; j < Q - 1; ++j) q[c][j] = q[c][j + 1];
+      q[c][Q - 1] = ql[c];
+#pragma unroll
+      for (int k = 0; k < RR; ++k) jr[c][k] = jr[c][k + 1];
+    }
   }
 }
 
@@ -230,6 +249,9 @@ __device__ __forceinline__ void load_point(const FArgs<R>& a, long long ic, R* p
   }
 }
 
+// Stage kernel body, lead-aligned like emf_tile: iteration `it` stages plane p = it + RR of B
+// (and E) into LDS and evaluates the x/y-derivative terms of the RHS of plane p; the output
+// plane `it` adds the z-derivative terms from the register queues and applies the RK update.
 template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ = false, int NBUF = 2>
 __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, int z0, StageLds<R, RR, TY, HALL, NBUF>& L) {
   constexpr int Q = 2 * RR + 1;
@@ -247,12 +269,14 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
   hsb.init(tx, ty, x0, y0, a.nx, a.ny, NX);
   if constexpr (HALL) hse.init(tx, ty, x0, y0, a.nx, a.ny, NX);
 
-  R qb[3][Q + 2];
-  R qe[2][HALL ? Q + 2 : 1];
-  R ez = R(0);  // E_z of the own column at the plane being staged
+  R qb[3][Q], qbl[3];                  // B of the own column: planes it-RR .. it+RR; landing it+RR+1
+  R qe[2][HALL ? Q : 1], qel[2];       // E_x, E_y of the own column (z-derivatives of curl E)
+  R ez = R(0);                         // E_z of the own column at the plane staged next
+  R rx[3][RR + 1];                     // x/y terms of the RHS for planes it .. it+RR
+  const int it0 = z0 - RR;
 #pragma unroll
-  for (int j = 0; j < Q + 1; ++j) {
-    const long long id = col + NXY * wrap(z0 - RR + j, a.nz);
+  for (int j = RR; j < Q; ++j) {  // planes z0-RR .. z0 (older slots are never read)
+    const long long id = col + NXY * wrap(it0 - RR + j, a.nz);
 #pragma unroll
     for (int c = 0; c < 3; ++c) qb[c][j] = B[c * S + id];
     if constexpr (HALL) {
@@ -265,149 +289,180 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
     hse.load(E, S, NXY * z0);
     ez = E[2 * S + col + NXY * z0];
   }
-  // pointwise RK operands of the current plane (pv, qv, gv) and of the next (pn, qn, gn)
+  // pointwise RK operands of the output plane (pv, qv, gv) and of the next one (pn, qn, gn)
   R pv[3], qv[3], gv[ADJ ? 3 : 1], pn[3], qn[3], gn[ADJ ? 3 : 1];
-  if (active) load_point<ADJ>(a, col + NXY * z0, pv, qv, gv);
-  for (int z = z0; z < zend; ++z) {
-    const int buf = NBUF == 2 ? (z - z0) & 1 : 0;
+  if (active) load_point<ADJ>(a, col + NXY * z0, pn, qn, gn);
+  for (int it = it0; it < zend; ++it) {
+    const int p = it + RR;
+    const int buf = NBUF == 2 ? (it - it0) & 1 : 0;
     if constexpr (NBUF == 1) {
-      if (z > z0) lds_barrier();
+      if (it > it0) lds_barrier();
     }
+    if (p < zend) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = qb[c][RR];
-    hsb.store(sB[buf], tx, ty);
-    if constexpr (HALL) {
-      sE[buf][0][ty + RR][tx + RR] = qe[0][RR];
-      sE[buf][1][ty + RR][tx + RR] = qe[1][RR];
-      sE[buf][2][ty + RR][tx + RR] = ez;
-      hse.store(sE[buf], tx, ty);
-    }
-    // Loads in the order they are consumed (the memory counter retires in issue order and
-    // also counts stores): halo of plane z+1 (next step), pointwise operands of plane z+1
-    // (end of next step), column plane z+RR+2 (two steps on).
-    if (z + 1 < zend) {
-      hsb.load(B, S, NXY * (z + 1));
+      for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = qb[c][Q - 1];
+      hsb.store(sB[buf], tx, ty);
       if constexpr (HALL) {
-        hse.load(E, S, NXY * (z + 1));
-        ez = E[2 * S + col + NXY * (z + 1)];
-      }
-      if (active) load_point<ADJ>(a, col + NXY * (z + 1), pn, qn, gn);
-    }
-    if (z + 2 < zend) {
-      const long long id = col + NXY * wrap(z + RR + 2, a.nz);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) qb[c][Q + 1] = B[c * S + id];
-      if constexpr (HALL) {
-        qe[0][Q + 1] = E[id];
-        qe[1][Q + 1] = E[S + id];
+        sE[buf][0][ty + RR][tx + RR] = qe[0][Q - 1];
+        sE[buf][1][ty + RR][tx + RR] = qe[1][Q - 1];
+        sE[buf][2][ty + RR][tx + RR] = ez;
+        hse.store(sE[buf], tx, ty);
       }
     }
-    const long long ic = col + NXY * z;
+    if (it >= z0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        pv[c] = pn[c];
+        qv[c] = qn[c];
+      }
+      if constexpr (ADJ) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gv[c] = gn[c];
+      }
+    }
+    // next iteration's loads in the order they are consumed (the memory counter retires in
+    // issue order and also counts stores): halo of plane p+1, pointwise operands of output
+    // plane it+1, column plane p+1
+    if (p + 1 < zend) {
+      hsb.load(B, S, NXY * (p + 1));
+      if constexpr (HALL) {
+        hse.load(E, S, NXY * (p + 1));
+        ez = E[2 * S + col + NXY * (p + 1)];
+      }
+    }
+    if (it + 1 > z0 && it + 1 < zend && active) load_point<ADJ>(a, col + NXY * (it + 1), pn, qn, gn);
+    if (it + 1 < zend) {
+      const long long id = col + NXY * wrap(p + 1, a.nz);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) qbl[c] = B[c * S + id];
+      if constexpr (HALL) {
+        qel[0] = E[id];
+        qel[1] = E[S + id];
+      }
+    }
     lds_barrier();
     if (active) {
-      R rhs[3] = {R(0), R(0), R(0)};
-      if constexpr (CONV) {
-        // uniform-velocity convection: sum_d (2/dx_d) 0.5 ([ZERO] u (D_d B)_d - u_d (D_d B))
+      if (p < zend) {  // x/y terms of plane p
+        R r[3] = {R(0), R(0), R(0)};
         R v[Q];
+        if constexpr (CONV) {
+          // uniform-velocity convection: sum_d (2/dx_d) 0.5 ([ZERO] u (D_d B)_d - u_d (D_d B))
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
+          for (int d = 0; d < 2; ++d) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+#pragma unroll
+              for (int k = 0; k < Q; ++k) v[k] = d == 0 ? sB[buf][c][ty + RR][tx + k] : sB[buf][c][ty + k][tx + RR];
+              const R der = deriv<R, RR>(a.c, v);
+              const R f = a.two_h[d] * R(0.5);
+              r[c] = r[c] - f * (a.u[d] * der);
+              if constexpr (ADJ) {
+                if (a.src_adj) r[d] = r[d] + f * (a.u[c] * der);
+              }
+              if (c == d && a.src_zero && !(ADJ && a.src_adj)) {
+#pragma unroll
+                for (int e = 0; e < 3; ++e) r[e] = r[e] + f * (a.u[e] * der);
+              }
+            }
+          }
+        }
+        if constexpr (HALL) {  // -curl_D E, x/y terms
+#pragma unroll
+          for (int k = 0; k < Q; ++k) v[k] = sE[buf][1][ty + RR][tx + k];
+          const R dxEy = deriv<R, RR>(a.c, v);
+#pragma unroll
+          for (int k = 0; k < Q; ++k) v[k] = sE[buf][2][ty + RR][tx + k];
+          const R dxEz = deriv<R, RR>(a.c, v);
+#pragma unroll
+          for (int k = 0; k < Q; ++k) v[k] = sE[buf][0][ty + k][tx + RR];
+          const R dyEx = deriv<R, RR>(a.c, v);
+#pragma unroll
+          for (int k = 0; k < Q; ++k) v[k] = sE[buf][2][ty + k][tx + RR];
+          const R dyEz = deriv<R, RR>(a.c, v);
+          r[0] = r[0] + a.inv_h[1] * (-dyEz);
+          r[1] = r[1] + a.inv_h[0] * dxEz;
+          r[2] = r[2] + (a.inv_h[0] * (-dxEy) + a.inv_h[1] * dyEx);
+        }
+        if constexpr (HOD) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            if (d == 0) {
 #pragma unroll
-              for (int k = 0; k < Q; ++k) v[k] = sB[buf][c][ty + RR][tx + k];
-            } else if (d == 1) {
+            for (int k = 0; k < Q; ++k) v[k] = sB[buf][c][ty + RR][tx + k];
+            R h = hodsum<R, RR>(a.c, v);
 #pragma unroll
-              for (int k = 0; k < Q; ++k) v[k] = sB[buf][c][ty + k][tx + RR];
-            } else {
+            for (int k = 0; k < Q; ++k) v[k] = sB[buf][c][ty + k][tx + RR];
+            h = h + hodsum<R, RR>(a.c, v);
+            r[c] = r[c] + h;
+          }
+        }
 #pragma unroll
-              for (int k = 0; k < Q; ++k) v[k] = qb[c][k];
-            }
-            const R der = deriv<R, RR>(a.c, v);
-            const R f = a.two_h[d] * R(0.5);
-            rhs[c] = rhs[c] - f * (a.u[d] * der);
+        for (int c = 0; c < 3; ++c) rx[c][RR] = r[c];
+      }
+      if (it >= z0) {  // output plane it: z terms and the RK update
+        R rhs[3] = {rx[0][0], rx[1][0], rx[2][0]};
+        if constexpr (CONV) {
+          const R f = a.two_h[2] * R(0.5);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const R der = deriv<R, RR>(a.c, qb[c]);
+            rhs[c] = rhs[c] - f * (a.u[2] * der);
             if constexpr (ADJ) {
-              if (a.src_adj) rhs[d] = rhs[d] + f * (a.u[c] * der);
+              if (a.src_adj) rhs[2] = rhs[2] + f * (a.u[c] * der);
             }
-            if (c == d && a.src_zero && !(ADJ && a.src_adj)) {
+            if (c == 2 && a.src_zero && !(ADJ && a.src_adj)) {
 #pragma unroll
               for (int e = 0; e < 3; ++e) rhs[e] = rhs[e] + f * (a.u[e] * der);
             }
           }
         }
-      }
-      if constexpr (HALL) {
-        R ev[Q];
-#pragma unroll
-        for (int k = 0; k < Q; ++k) ev[k] = sE[buf][1][ty + RR][tx + k];
-        const R dxEy = deriv<R, RR>(a.c, ev);
-#pragma unroll
-        for (int k = 0; k < Q; ++k) ev[k] = sE[buf][2][ty + RR][tx + k];
-        const R dxEz = deriv<R, RR>(a.c, ev);
-#pragma unroll
-        for (int k = 0; k < Q; ++k) ev[k] = sE[buf][0][ty + k][tx + RR];
-        const R dyEx = deriv<R, RR>(a.c, ev);
-#pragma unroll
-        for (int k = 0; k < Q; ++k) ev[k] = sE[buf][2][ty + k][tx + RR];
-        const R dyEz = deriv<R, RR>(a.c, ev);
-        const R dzEx = deriv<R, RR>(a.c, qe[0]);
-        const R dzEy = deriv<R, RR>(a.c, qe[1]);
-        rhs[0] = rhs[0] + (a.inv_h[1] * (-dyEz) + a.inv_h[2] * dzEy);
-        rhs[1] = rhs[1] + (a.inv_h[0] * dxEz + a.inv_h[2] * (-dzEx));
-        rhs[2] = rhs[2] + (a.inv_h[0] * (-dxEy) + a.inv_h[1] * dyEx);
-      }
-      if constexpr (HOD) {
-        R v[Q];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-#pragma unroll
-          for (int k = 0; k < Q; ++k) v[k] = sB[buf][c][ty + RR][tx + k];
-          R h = hodsum<R, RR>(a.c, v);
-#pragma unroll
-          for (int k = 0; k < Q; ++k) v[k] = sB[buf][c][ty + k][tx + RR];
-          h = h + hodsum<R, RR>(a.c, v);
-          h = h + hodsum<R, RR>(a.c, qb[c]);
-          rhs[c] = rhs[c] + h;
+        if constexpr (HALL) {
+          const R dzEx = deriv<R, RR>(a.c, qe[0]);
+          const R dzEy = deriv<R, RR>(a.c, qe[1]);
+          rhs[0] = rhs[0] + a.inv_h[2] * dzEy;
+          rhs[1] = rhs[1] + a.inv_h[2] * (-dzEx);
         }
-      }
-      if constexpr (ADJ) {
+        if constexpr (HOD) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) rhs[c] = rhs[c] - gv[c];
-      }
-      if (a.mode == 0) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          R r = a.cD * rhs[c];
-          if constexpr (ADJ) r = a.cY * qb[c][RR] + r;
-          if (a.Q) r = a.cQ * qv[c] + r;
-          if (a.P) r = a.cP * pv[c] + r;
-          a.out[c * S + ic] = r;
+          for (int c = 0; c < 3; ++c) rhs[c] = rhs[c] + hodsum<R, RR>(a.c, qb[c]);
         }
-      } else {
+        if constexpr (ADJ) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const R b2 = a.cP * pv[c] + a.cD * rhs[c];
-          a.b2[c * S + ic] = b2;
-          a.out[c * S + ic] = qb[c][RR] + a.cQ * b2;
+          for (int c = 0; c < 3; ++c) rhs[c] = rhs[c] - gv[c];
+        }
+        const long long ic = col + NXY * it;
+        if (a.mode == 0) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            R o = a.cD * rhs[c];
+            if constexpr (ADJ) o = a.cY * qb[c][RR] + o;
+            if (a.Q) o = a.cQ * qv[c] + o;
+            if (a.P) o = a.cP * pv[c] + o;
+            a.out[c * S + ic] = o;
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const R b2 = a.cP * pv[c] + a.cD * rhs[c];
+            a.b2[c * S + ic] = b2;
+            a.out[c * S + ic] = qb[c][RR] + a.cQ * b2;
+          }
         }
       }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      pv[c] = pn[c];
-      qv[c] = qn[c];
+#pragma unroll
+      for (int j = 0; j < Q - 1; ++j) qb[c][j] = qb[c][j + 1];
+      qb[c][Q - 1] = qbl[c];
+#pragma unroll
+      for (int k = 0; k < RR; ++k) rx[c][k] = rx[c][k + 1];
     }
-    if constexpr (ADJ) {
+    if constexpr (HALL) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) gv[c] = gn[c];
-    }
+      for (int c = 0; c < 2; ++c) {
 #pragma unroll
-    for (int j = 0; j < Q + 1; ++j) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) qb[c][j] = qb[c][j + 1];
-      if constexpr (HALL) {
-        qe[0][j] = qe[0][j + 1];
-        qe[1][j] = qe[1][j + 1];
+        for (int j = 0; j < Q - 1; ++j) qe[c][j] = qe[c][j + 1];
+        qe[c][Q - 1] = qel[c];
       }
     }
   }
