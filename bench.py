@@ -41,6 +41,9 @@ def parse():
                     help="hall128: one periodic Hall solve (the headline metric); ensemble64: 64 batched FP32 solves "
                          "against the same solves run one by one in FP64")
     ap.add_argument("--members", type=int, default=64, help="ensemble64: number of members")
+    ap.add_argument("--traffic-from", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="only summarise rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE counter_collection.csv files of this "
+                         "benchmark and of --copy-probe into HBM bytes per launch (JSON on stdout)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -59,10 +62,51 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
 
 
+def _counter_means(path, counter):
+    import csv
+    import re
+
+    acc = {}
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        name = r["Kernel_Name"].replace("void ", "", 1).replace("ieq::(anonymous namespace)::", "")
+        m = re.match(r"([\w:]+(?:<[^()]*>)?)", name)
+        k = m.group(1) if m else name
+        acc.setdefault(k, []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def traffic_summary(fetch_csv, write_csv, copyf_csv, copyw_csv, copy_mib=512):
+    """HBM-side bytes per dispatch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the same
+    counters read for a plain device copy of copy_mib MiB (bench.py --copy-probe)."""
+    f, w = _counter_means(fetch_csv, "FETCH_SIZE"), _counter_means(write_csv, "WRITE_SIZE")
+    cf = max(_counter_means(copyf_csv, "FETCH_SIZE").values())
+    cw = max(_counter_means(copyw_csv, "WRITE_SIZE").values())
+    kib = copy_mib * 1024.0
+    ff, wf = kib / cf, kib / cw
+    kern = {k: {"fetch_kib_raw": f[k], "write_kib_raw": w.get(k, 0.0), "bytes": (f[k] * ff + w.get(k, 0.0) * wf) * 1024}
+            for k in f if k.startswith("k_")}
+    return {"fetch_factor": ff, "write_factor": wf, "kernels": kern}
+
+
+def stored_traffic(name):
+    """Traffic per RK stage (k_emf + k_stage dispatches) from a committed summary, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    return {"bytes_per_stage": sum(v["bytes"] for k, v in d["kernels"].items() if k.startswith(("k_emf", "k_stage"))),
+            "source": "profiles/" + name}
+
+
 def main():
     a = parse()
     if a.copy_probe:
         copy_probe(a.copy_probe)
+        return
+    if a.traffic_from:
+        print(json.dumps(traffic_summary(*a.traffic_from), indent=1))
         return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the workload is single-GPU; multi-GPU domain "
@@ -128,6 +172,9 @@ def main():
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(cfg, B0, a.integrator, a.cpu_steps, stages)
+    traffic = None
+    if n == 128 and a.integrator == "CarpenterKennedy2N54":
+        traffic = stored_traffic("hall128_%s_traffic.json" % a.precision)
 
     out = {
         "metric": "Mcell-updates/s per RK stage",
@@ -146,7 +193,9 @@ def main():
                    "order": 6, "dissipation": "high_order", "u": [0, 0, 0], "hall_coefficient": 1.0, "dt": cfg.dt,
                    "stages_per_step": stages, "seed": 1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic["bytes_per_stage"]) if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "kernel": "one fused RK stage (k_emf + k_stage), %d B/cell algorithmic" % (4 * 3 * rb),
                      "us_per_stage": round(stage_s * 1e6, 2)},
         "cpu_baseline": cpu,
