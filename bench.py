@@ -375,8 +375,18 @@ def bench_adjoint(a):
             ad["c_hall"] = gc
             res[real]["fd"] = {k: {"fd": fd[k], "adjoint": ad[k], "rel_err": abs(fd[k] - ad[k]) / abs(fd[k])}
                                for k in fd}
+        if real == "float":  # the same gradient with every stage input kept (no recomputation)
+            full = AdjointSolver(prog, steps, steps)
+            full.forward(tgt0, u, ch)
+            bT2 = full.forward(b0, u, ch)
+            g2, gc2, gu2 = full.backward((bT2 - tgt) * (h ** 3))
+            res["float_stored"] = {"t_fwd": full.last_forward_seconds, "t_bwd": full.last_backward_seconds,
+                                   "counts": full.counts(),
+                                   "same_gradient": bool(torch.equal(g2, g)) and gc2 == gc and gu2 == gu}
+            del full
         del solver, prog
     r32, r64 = res["float"], res["double"]
+    rs = res["float_stored"]
     gerr = float((r32["grad"] - r64["grad"]).abs().max() / r64["grad"].abs().max())
     ratio = (r32["t_fwd"] + r32["t_bwd"]) / r32["t_plain"]
     stages = 3 * steps
@@ -399,6 +409,10 @@ def bench_adjoint(a):
         "timings_s": {k: {"plain_forward": round(v["t_plain"], 5), "checkpointed_forward": round(v["t_fwd"], 5),
                           "backward": round(v["t_bwd"], 5), "forward_steps": v["counts"][0],
                           "adjoint_steps": v["counts"][1]} for k, v in res.items()},
+        "all_stages_stored": {"cost_x_forward": round((rs["t_fwd"] + rs["t_bwd"]) / r32["t_plain"], 3),
+                              "checkpointed_forward": round(rs["t_fwd"], 5), "backward": round(rs["t_bwd"], 5),
+                              "forward_steps": rs["counts"][0], "adjoint_steps": rs["counts"][1],
+                              "bitwise_same_gradient": rs["same_gradient"]},
         "fp32_vs_fp64_grad_rel_err": gerr,
         "fd_check_fp64": r64["fd"],
         "cpu_baseline": None,

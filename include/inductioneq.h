@@ -189,7 +189,8 @@ void ieq_ensemble_release(ieq_ensemble *ens);
  * compute_numerical_solution.m:91-108 run with the SSPRK33 kernels
  * (kernel_time_integrator.cl:224-303).  `snapshots` vector fields are kept for
  * optimal binomial checkpointing of the nsteps steps (stage inputs are recomputed from the
- * step start); snapshots >= nsteps stores every step. */
+ * step start); snapshots >= nsteps keeps every step's three stage inputs (3 nsteps + 1
+ * vector fields, when they fit in half the free device memory) and recomputes nothing. */
 typedef struct ieq_adjoint ieq_adjoint;
 int ieq_adjoint_create(ieq_adjoint **adj, ieq_program *prog, long long nsteps, int snapshots);
 

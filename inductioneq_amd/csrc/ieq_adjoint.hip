@@ -249,7 +249,13 @@ struct Adjoint {
   double c_hall = 1.0, u[3] = {0, 0, 0};
   bool have_forward = false;
   long long fwd_steps = 0, adj_steps = 0;
-  std::vector<std::pair<long long, int>> chain;  // (step, slot) stored by the forward sweep
+  // snapshots >= nsteps: every step's stage inputs b1, b2, b3 are kept (3 nsteps + 1 fields)
+  // and the backward recomputes nothing
+  bool stages = false;
+  void* store = nullptr;
+  void* st(int which, long long k) const {  // which: 0 b1, 1 b2, 2 b3
+    return (char*)store + (size_t)(which == 0 ? k : nsteps + 1 + (which - 1) * nsteps + k) * vbytes;
+  }
 };
 
 namespace {
@@ -281,14 +287,15 @@ template <typename R> StencilCoef<R> coefs(const Adjoint& A) {
   return c;
 }
 
-// One forward SSPRK33 step in place on `b` (registers b2, b3, E); the reference stage arithmetic.
-template <typename R> void fwd_step(Adjoint& A, R* b) {
+// One forward SSPRK33 step from b1 with stage registers b2, b3 (and E); the last stage writes
+// `out` (which may be b1 itself).  The reference stage arithmetic (kernel_time_integrator.cl:224-303).
+template <typename R> void fwd_step_to(Adjoint& A, R* b1, R* b2, R* b3, R* out) {
   const StencilCoef<R> coef = coefs<R>(A);
   FArgs<R> a = base_args<R>(A, coef);
-  R* F[3] = {b, (R*)A.b2, (R*)A.b3};
+  R* F[4] = {b1, b2, b3, out};
   const bool hall = A.p.hall, hod = A.p.diss == 1;
-  // (out, P, cP, Q, cQ, Y, cD) of SSPRK33 (kernel_time_integrator.cl:224-303)
-  const int outs[3] = {1, 2, 0}, Ps[3] = {0, 0, 0}, Qs[3] = {-1, 1, 2}, Ys[3] = {0, 1, 2};
+  // (out, P, cP, Q, cQ, Y, cD) of the three stages
+  const int outs[3] = {1, 2, 3}, Ps[3] = {0, 0, 0}, Qs[3] = {-1, 1, 2}, Ys[3] = {0, 1, 2};
   const double cPs[3] = {1.0, 0.75, 1.0 / 3.0}, cQs[3] = {0.0, 0.25, 2.0 / 3.0}, cDs[3] = {1.0, 0.25, 2.0 / 3.0};
   for (int k = 0; k < 3; ++k) {
     if (hall) {
@@ -309,6 +316,8 @@ template <typename R> void fwd_step(Adjoint& A, R* b) {
   }
   ++A.fwd_steps;
 }
+
+template <typename R> void fwd_step(Adjoint& A, R* b) { fwd_step_to<R>(A, b, (R*)A.b2, (R*)A.b3, b); }
 
 template <typename R> void pre(Adjoint& A, const R* Y, const R* L, double scale) {
   PreArgs<R> q{};
@@ -363,9 +372,26 @@ template <typename R> void adj_stage(Adjoint& A, const R* L, R* out, const R* P,
   launch_stage<R>(A.rr, kAdjTY, a, A.p.hall != 0, A.p.diss == 1, A.s);
 }
 
-// Reverse one SSPRK33 step whose start state is b1 (read only): lam <- step^T lam, parameter partials.
+// Reverse one SSPRK33 step given its stage inputs b1, b2, b3: lam <- step^T lam, parameter partials.
+template <typename R> void adj_from(Adjoint& A, const R* b1, const R* b2, const R* b3) {
+  R* lam = (R*)A.lam;
+  R* lam3 = (R*)A.lam3;
+  R* lam2 = (R*)A.lam2;
+  // stage 3: b1' = 1/3 b1 + 2/3 b3 + 2/3 dt R(b3)
+  pre<R>(A, b3, lam, 2.0 / 3.0 * A.p.dt);
+  adj_stage<R>(A, lam, lam3, lam, 2.0 / 3.0, nullptr, 0.0, 0.0, 2.0 / 3.0);  // lam3 = 2/3 lam + 2/3 dt R^T lam
+  // stage 2: b3 = 3/4 b1 + 1/4 b2 + 1/4 dt R(b2)
+  pre<R>(A, b2, lam3, 0.25 * A.p.dt);
+  adj_stage<R>(A, lam3, lam2, lam3, 0.25, nullptr, 0.0, 0.0, 0.25);  // lam2 = 1/4 lam3 + 1/4 dt R^T lam3
+  // stage 1: b2 = b1 + dt R(b1); lam_b1 = 1/3 lam + 3/4 lam3 + lam2 + dt R^T lam2
+  pre<R>(A, b1, lam2, A.p.dt);
+  adj_stage<R>(A, lam2, lam, lam, 1.0 / 3.0, lam3, 0.75, 1.0, 1.0);
+  ++A.adj_steps;
+}
+
+// Reverse one step whose start state is b1 (read only): recompute b2, b3 (the first two
+// stages), then the transposed stages.
 template <typename R> void adj_step(Adjoint& A, const R* b1) {
-  // recompute the stage inputs b2, b3 from b1
   const StencilCoef<R> coef = coefs<R>(A);
   FArgs<R> a = base_args<R>(A, coef);
   const bool hall = A.p.hall, hod = A.p.diss == 1;
@@ -386,19 +412,7 @@ template <typename R> void adj_step(Adjoint& A, const R* b1) {
   }
   a.Y = b2; a.E = (R*)A.E; a.out = b3; a.P = b1; a.Q = b2; a.cP = (R)0.75; a.cQ = (R)0.25; a.cD = (R)(0.25 * A.p.dt);
   launch_stage<R>(A.rr, kAdjTY, a, hall, hod, A.s);
-  R* lam = (R*)A.lam;
-  R* lam3 = (R*)A.lam3;
-  R* lam2 = (R*)A.lam2;
-  // stage 3: b1' = 1/3 b1 + 2/3 b3 + 2/3 dt R(b3)
-  pre<R>(A, b3, lam, 2.0 / 3.0 * A.p.dt);
-  adj_stage<R>(A, lam, lam3, lam, 2.0 / 3.0, nullptr, 0.0, 0.0, 2.0 / 3.0);  // lam3 = 2/3 lam + 2/3 dt R^T lam
-  // stage 2: b3 = 3/4 b1 + 1/4 b2 + 1/4 dt R(b2)
-  pre<R>(A, b2, lam3, 0.25 * A.p.dt);
-  adj_stage<R>(A, lam3, lam2, lam3, 0.25, nullptr, 0.0, 0.0, 0.25);  // lam2 = 1/4 lam3 + 1/4 dt R^T lam3
-  // stage 1: b2 = b1 + dt R(b1); lam_b1 = 1/3 lam + 3/4 lam3 + lam2 + dt R^T lam2
-  pre<R>(A, b1, lam2, A.p.dt);
-  adj_stage<R>(A, lam2, lam, lam, 1.0 / 3.0, lam3, 0.75, 1.0, 1.0);
-  ++A.adj_steps;
+  adj_from<R>(A, b1, b2, b3);
 }
 
 template <typename R> void copy(Adjoint& A, void* dst, const void* src) {
@@ -448,6 +462,13 @@ template <typename R> void reverse(Adjoint& A, long long i, long long n, int si,
 }
 
 template <typename R> int forward_t(Adjoint& A, const void* b0, void* bT) {
+  if (A.stages) {
+    copy<R>(A, A.st(0, 0), b0);
+    for (long long k = 0; k < A.nsteps; ++k)
+      fwd_step_to<R>(A, (R*)A.st(0, k), (R*)A.st(1, k), (R*)A.st(2, k), (R*)A.st(0, k + 1));
+    copy<R>(A, bT, A.st(0, A.nsteps));
+    return 0;
+  }
   for (size_t k = 1; k < A.slots.size(); ++k) A.slot_step[k] = -1;
   copy<R>(A, A.slots[0], b0);
   A.slot_step[0] = 0;
@@ -482,7 +503,12 @@ template <typename R> int forward_t(Adjoint& A, const void* b0, void* bT) {
 template <typename R> int backward_t(Adjoint& A, const void* lamT, void* grad_b0) {
   copy<R>(A, A.lam, lamT);
   k_adj_zero<<<1, 64, 0, A.s>>>(A.acc);
-  reverse<R>(A, 0, A.nsteps, 0, A.snapshots);
+  if (A.stages) {
+    for (long long k = A.nsteps - 1; k >= 0; --k)
+      adj_from<R>(A, (const R*)A.st(0, k), (const R*)A.st(1, k), (const R*)A.st(2, k));
+  } else {
+    reverse<R>(A, 0, A.nsteps, 0, A.snapshots);
+  }
   copy<R>(A, grad_b0, A.lam);
   return 0;
 }
@@ -521,6 +547,14 @@ int adjoint_create(const Params& p, const ieq_program_info& info, long long nste
       adjoint_release(A);
       return err = "hipMalloc failed (adjoint registers)", IEQ_ERR_HIP;
     }
+  if (snapshots >= nsteps) {  // keep every stage input when the device has room for it
+    size_t freeb = 0, totalb = 0;
+    const size_t need = (size_t)(3 * nsteps + 1) * A->vbytes;
+    if (hipMemGetInfo(&freeb, &totalb) == hipSuccess && need < freeb / 2 && hipMalloc(&A->store, need) == hipSuccess) {
+      A->stages = true;
+      A->snapshots = 0;  // the snapshot pool below only holds the start state
+    }
+  }
   A->slots.assign(A->snapshots + 1, nullptr);
   A->slot_step.assign(A->snapshots + 1, -1);
   for (auto& sl : A->slots)
@@ -602,6 +636,7 @@ void adjoint_release(Adjoint* A) {
     if (sl) (void)hipFree(sl);
   if (A->partial) (void)hipFree(A->partial);
   if (A->acc) (void)hipFree(A->acc);
+  if (A->store) (void)hipFree(A->store);
   delete A;
 }
 
