@@ -58,8 +58,10 @@ __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
   const long long col = wrap(x0 + tx, a.nx) + NX * wrap(y0 + ty, a.ny);
   const int zend = min(z0 + a.zc, a.nz);
   PlaneStage<R, RR, TY, 3> hy, hl;
-  hy.init(tx, ty, x0, y0, a.nx, a.ny, NX);
-  hl.init(tx, ty, x0, y0, a.nx, a.ny, NX);
+  hy.init(tx, ty, x0, y0, a.nx, a.ny);
+  hl.init(tx, ty, x0, y0, a.nx, a.ny);
+  const Comps<R, 3> Yc = comps<R, 3>(a.Y, S), Lc = comps<R, 3>(a.L, S);
+  const unsigned plb = (unsigned)(a.nx * a.ny) * (unsigned)sizeof(R);
 
   R qy[3][Q], ql[2][Q];  // z-queues: Y (all components), L (x, y components)
 #pragma unroll
@@ -86,8 +88,8 @@ __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
     ql[1][Q - 1] = a.L[S + idn];
     const long long ic = col + NXY * z;
     const R lz = a.L[2 * S + ic];
-    hy.load(a.Y, S, NXY * z);
-    hl.load(a.L, S, NXY * z);
+    hy.load(Yc, (unsigned)z * plb);
+    hl.load(Lc, (unsigned)z * plb);
     __syncthreads();  // previous plane's LDS reads are done
 #pragma unroll
     for (int c = 0; c < 3; ++c) sY[c][ty + RR][tx + RR] = qy[c][RR];
@@ -530,6 +532,10 @@ int adjoint_create(const Params& p, const ieq_program_info& info, long long nste
   if (A->rr < 1 || A->rr > 3) {
     delete A;
     return err = "the adjoint needs an operator whose central rows are at most 7 wide", IEQ_ERR_UNSUPPORTED;
+  }
+  if (!offsets_fit(p.nx, p.ny, p.nz, info.real_bytes)) {
+    delete A;
+    return err = "the grid is too large for the adjoint (one component must stay below 2 GiB)", IEQ_ERR_UNSUPPORTED;
   }
   A->nsteps = nsteps;
   A->snapshots = (int)std::min<long long>(snapshots, nsteps);

@@ -121,12 +121,13 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
   constexpr int Q = 2 * RR + 1;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const bool active = x0 + tx < a.nx && y0 + ty < a.ny;
-  const long long S = a.S, NX = a.nx, NXY = (long long)a.nx * a.ny;
-  const long long col = wrap(x0 + tx, a.nx) + NX * wrap(y0 + ty, a.ny);
+  const unsigned RB = sizeof(R);
+  const unsigned colb = (unsigned)(wrap(x0 + tx, a.nx) + a.nx * wrap(y0 + ty, a.ny)) * RB;  // own column
+  const unsigned plb = (unsigned)(a.nx * a.ny) * RB;                                       // plane stride
   const int zend = min(z0 + a.zc, a.nz);
-  const R* B = a.Y;
+  const Comps<R, 3> B = comps<R, 3>(a.Y, a.S);
   PlaneStage<R, RR, TY, 3> hs;
-  hs.init(tx, ty, x0, y0, a.nx, a.ny, NX);
+  hs.init(tx, ty, x0, y0, a.nx, a.ny);
 
   R q[3][Q];       // own column: planes it-RR .. it+RR
   R ql[3];         // landing register: plane it+RR+1
@@ -134,11 +135,11 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
   const int it0 = z0 - RR;
 #pragma unroll
   for (int j = RR; j < Q; ++j) {  // planes z0-RR .. z0 (older slots are never read)
-    const long long id = col + NXY * wrap(it0 - RR + j, a.nz);
+    const unsigned o = colb + (unsigned)wrap(it0 - RR + j, a.nz) * plb;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) q[c][j] = B[c * S + id];
+    for (int c = 0; c < 3; ++c) q[c][j] = ldo(B.c[c], o);
   }
-  hs.load(B, S, NXY * z0);
+  hs.load(B, (unsigned)z0 * plb);
   for (int it = it0; it < zend; ++it) {
     const int p = it + RR;  // plane staged in this iteration
     const int buf = NBUF == 2 ? (it - it0) & 1 : 0;
@@ -151,11 +152,11 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
       hs.store(sB[buf], tx, ty);
     }
     // next iteration's loads, halo first (the memory counter retires in issue order)
-    if (p + 1 < zend) hs.load(B, S, NXY * (p + 1));
+    if (p + 1 < zend) hs.load(B, (unsigned)(p + 1) * plb);
     if (it + 1 < zend) {
-      const long long id = col + NXY * wrap(p + 1, a.nz);
+      const unsigned o = colb + (unsigned)wrap(p + 1, a.nz) * plb;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) ql[c] = B[c * S + id];
+      for (int c = 0; c < 3; ++c) ql[c] = ldo(B.c[c], o);
     }
     lds_barrier();
     if (active) {
@@ -183,10 +184,10 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
         // J = curl B / rho  (SBP_operator.cl:126-135, kernel_time_integrator.cl:147)
         const R Jx = (jr[0][0] - dzBy) / a.rho, Jy = (dzBx + jr[1][0]) / a.rho, Jz = jr[2][0] / a.rho;
         const R Bx = q[0][RR], By = q[1][RR], Bz = q[2][RR];
-        const long long ic = col + NXY * it;
-        a.Eout[ic] = Jy * Bz - Jz * By;
-        a.Eout[S + ic] = Jz * Bx - Jx * Bz;
-        a.Eout[2 * S + ic] = Jx * By - Jy * Bx;
+        const unsigned o = colb + (unsigned)it * plb;
+        sto(a.Eout, o, Jy * Bz - Jz * By);
+        sto(a.Eout + a.S, o, Jz * Bx - Jx * Bz);
+        sto(a.Eout + 2 * a.S, o, Jx * By - Jy * Bx);
       }
     }
 #pragma unroll
@@ -236,16 +237,17 @@ template <typename R, int RR, int TY, bool HALL, int NBUF = 2> struct StageLds {
 
 // Pointwise operands of the RK update at node ic: P (or b2 in 2N mode), Q, and the adjoint's G.
 template <bool ADJ, typename R>
-__device__ __forceinline__ void load_point(const FArgs<R>& a, long long ic, R* pv, R* qv, R* gv) {
+__device__ __forceinline__ void load_point(const FArgs<R>& a, unsigned o, R* pv, R* qv, R* gv) {
   const long long S = a.S;
+  const R* P = a.mode == 0 ? a.P : a.b2;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    pv[c] = a.mode == 0 ? (a.P ? a.P[c * S + ic] : R(0)) : a.b2[c * S + ic];
-    qv[c] = (a.mode == 0 && a.Q) ? a.Q[c * S + ic] : R(0);
+    pv[c] = P ? ldo(P + c * S, o) : R(0);
+    qv[c] = (a.mode == 0 && a.Q) ? ldo(a.Q + c * S, o) : R(0);
   }
   if constexpr (ADJ) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) gv[c] = a.G ? a.G[c * S + ic] : R(0);
+    for (int c = 0; c < 3; ++c) gv[c] = a.G ? ldo(a.G + c * S, o) : R(0);
   }
 }
 
@@ -259,15 +261,17 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
   auto& sE = L.sE;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const bool active = x0 + tx < a.nx && y0 + ty < a.ny;
-  const long long S = a.S, NX = a.nx, NXY = (long long)a.nx * a.ny;
-  const long long col = wrap(x0 + tx, a.nx) + NX * wrap(y0 + ty, a.ny);
+  const long long S = a.S;
+  const unsigned RB = sizeof(R);
+  const unsigned colb = (unsigned)(wrap(x0 + tx, a.nx) + a.nx * wrap(y0 + ty, a.ny)) * RB;  // own column
+  const unsigned plb = (unsigned)(a.nx * a.ny) * RB;                                       // plane stride
   const int zend = min(z0 + a.zc, a.nz);
-  const R* B = a.Y;
-  const R* E = a.E;
+  const Comps<R, 3> B = comps<R, 3>(a.Y, S);
+  const Comps<R, 3> E = comps<R, 3>(HALL ? a.E : a.Y, S);
   PlaneStage<R, RR, TY, 3> hsb;
   PlaneStage<R, RR, TY, 3> hse;
-  hsb.init(tx, ty, x0, y0, a.nx, a.ny, NX);
-  if constexpr (HALL) hse.init(tx, ty, x0, y0, a.nx, a.ny, NX);
+  hsb.init(tx, ty, x0, y0, a.nx, a.ny);
+  if constexpr (HALL) hse.init(tx, ty, x0, y0, a.nx, a.ny);
 
   R qb[3][Q], qbl[3];                  // B of the own column: planes it-RR .. it+RR; landing it+RR+1
   R qe[2][HALL ? Q : 1], qel[2];       // E_x, E_y of the own column (z-derivatives of curl E)
@@ -276,22 +280,22 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
   const int it0 = z0 - RR;
 #pragma unroll
   for (int j = RR; j < Q; ++j) {  // planes z0-RR .. z0 (older slots are never read)
-    const long long id = col + NXY * wrap(it0 - RR + j, a.nz);
+    const unsigned o = colb + (unsigned)wrap(it0 - RR + j, a.nz) * plb;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) qb[c][j] = B[c * S + id];
+    for (int c = 0; c < 3; ++c) qb[c][j] = ldo(B.c[c], o);
     if constexpr (HALL) {
-      qe[0][j] = E[id];
-      qe[1][j] = E[S + id];
+      qe[0][j] = ldo(E.c[0], o);
+      qe[1][j] = ldo(E.c[1], o);
     }
   }
-  hsb.load(B, S, NXY * z0);
+  hsb.load(B, (unsigned)z0 * plb);
   if constexpr (HALL) {
-    hse.load(E, S, NXY * z0);
-    ez = E[2 * S + col + NXY * z0];
+    hse.load(E, (unsigned)z0 * plb);
+    ez = ldo(E.c[2], colb + (unsigned)z0 * plb);
   }
   // pointwise RK operands of the output plane (pv, qv, gv) and of the next one (pn, qn, gn)
   R pv[3], qv[3], gv[ADJ ? 3 : 1], pn[3], qn[3], gn[ADJ ? 3 : 1];
-  if (active) load_point<ADJ>(a, col + NXY * z0, pn, qn, gn);
+  if (active) load_point<ADJ>(a, colb + (unsigned)z0 * plb, pn, qn, gn);
   for (int it = it0; it < zend; ++it) {
     const int p = it + RR;
     const int buf = NBUF == 2 ? (it - it0) & 1 : 0;
@@ -324,20 +328,20 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
     // issue order and also counts stores): halo of plane p+1, pointwise operands of output
     // plane it+1, column plane p+1
     if (p + 1 < zend) {
-      hsb.load(B, S, NXY * (p + 1));
+      hsb.load(B, (unsigned)(p + 1) * plb);
       if constexpr (HALL) {
-        hse.load(E, S, NXY * (p + 1));
-        ez = E[2 * S + col + NXY * (p + 1)];
+        hse.load(E, (unsigned)(p + 1) * plb);
+        ez = ldo(E.c[2], colb + (unsigned)(p + 1) * plb);
       }
     }
-    if (it + 1 > z0 && it + 1 < zend && active) load_point<ADJ>(a, col + NXY * (it + 1), pn, qn, gn);
+    if (it + 1 > z0 && it + 1 < zend && active) load_point<ADJ>(a, colb + (unsigned)(it + 1) * plb, pn, qn, gn);
     if (it + 1 < zend) {
-      const long long id = col + NXY * wrap(p + 1, a.nz);
+      const unsigned o = colb + (unsigned)wrap(p + 1, a.nz) * plb;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) qbl[c] = B[c * S + id];
+      for (int c = 0; c < 3; ++c) qbl[c] = ldo(B.c[c], o);
       if constexpr (HALL) {
-        qel[0] = E[id];
-        qel[1] = E[S + id];
+        qel[0] = ldo(E.c[0], o);
+        qel[1] = ldo(E.c[1], o);
       }
     }
     lds_barrier();
@@ -429,7 +433,7 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
 #pragma unroll
           for (int c = 0; c < 3; ++c) rhs[c] = rhs[c] - gv[c];
         }
-        const long long ic = col + NXY * it;
+        const unsigned oc = colb + (unsigned)it * plb;
         if (a.mode == 0) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -437,14 +441,14 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
             if constexpr (ADJ) o = a.cY * qb[c][RR] + o;
             if (a.Q) o = a.cQ * qv[c] + o;
             if (a.P) o = a.cP * pv[c] + o;
-            a.out[c * S + ic] = o;
+            sto(a.out + c * S, oc, o);
           }
         } else {
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             const R b2 = a.cP * pv[c] + a.cD * rhs[c];
-            a.b2[c * S + ic] = b2;
-            a.out[c * S + ic] = qb[c][RR] + a.cQ * b2;
+            sto(a.b2 + c * S, oc, b2);
+            sto(a.out + c * S, oc, qb[c][RR] + a.cQ * b2);
           }
         }
       }
@@ -763,6 +767,7 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
   StencilCoef<R> coef;
   const int rr = central_coefs<R>(info.opset, p.ho_factor, coef);
   if (rr < 1 || rr > 3) return IEQ_OK;
+  if (!offsets_fit(p.nx, p.ny, p.nz, sizeof(R))) return IEQ_OK;  // 32-bit byte offsets per component
 
   R* F[3] = {(R*)args[0], (R*)args[1], nf == 3 ? (R*)args[2] : nullptr};
   R* J = (R*)args[nf];
@@ -1187,6 +1192,10 @@ int ensemble_create(const Params& p, const ieq_program_info& info, const ieq_mem
         !(mb.dy > 0) || !(mb.dz > 0)) {
       delete e;
       return err = "invalid member " + std::to_string(i), IEQ_ERR_ARGS;
+    }
+    if (!offsets_fit(mb.nx, mb.ny, mb.nz, info.real_bytes)) {
+      delete e;
+      return err = "member " + std::to_string(i) + " is too large (one component must stay below 2 GiB)", IEQ_ERR_UNSUPPORTED;
     }
     if (mb.ux != 0.0 || mb.uy != 0.0 || mb.uz != 0.0) e->conv = 1;
     e->m.push_back(mb);

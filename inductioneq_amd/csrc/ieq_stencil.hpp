@@ -93,6 +93,32 @@ __device__ inline void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Global accesses of the fused kernels use a uniform base pointer per field component
+// (held in SGPRs) and a 32-bit per-lane byte offset, which maps onto the SGPR-base form of
+// the global load/store instructions (no 64-bit address arithmetic per access).  The host
+// only takes these paths when one component plane of a field is below 2^31 bytes.
+template <typename R> __device__ __forceinline__ R ldo(const R* __restrict__ base, unsigned off) {
+  return *(const R*)((const char*)base + off);
+}
+template <typename R> __device__ __forceinline__ void sto(R* __restrict__ base, unsigned off, R v) {
+  *(R*)((char*)base + off) = v;
+}
+// Host check: one component plane of an nx*ny*nz field of `rb`-byte REALs is addressable with
+// 32-bit byte offsets.
+inline bool offsets_fit(long long nx, long long ny, long long nz, long long rb) {
+  return nx * ny * nz * rb < (1LL << 31);
+}
+
+template <typename R, int NC> struct Comps {  // component base pointers of a field
+  const R* c[NC];
+};
+template <typename R, int NC> __device__ __forceinline__ Comps<R, NC> comps(const R* f, long long S) {
+  Comps<R, NC> r;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) r.c[k] = f + k * S;
+  return r;
+}
+
 // Per-thread staging of one xy-plane of an NC-component field into an LDS tile with a
 // halo of RR cells.  The thread owns its column (x0+tx, y0+ty) and, at the tile edges,
 // up to one x-halo cell and up to two y-halo cells (all coordinates wrap periodically).
@@ -102,35 +128,36 @@ template <typename R, int RR, int TY, int NC> struct PlaneStage {
   static constexpr bool ONE_Y = TY >= 2 * RR;
   bool hx, ht, hb;
   int lcx, lry;                // LDS column of the x-halo cell; LDS row of the y-halo cell (ONE_Y)
-  long long gx, gt, gb;        // in-plane global indices of the halo cells
+  unsigned gx, gt, gb;         // in-plane byte offsets of the halo cells
   R vx[NC], vt[NC], vb[ONE_Y ? 1 : NC];  // prefetched halo values of the next plane to stage
 
-  __device__ void init(int tx, int ty, int x0, int y0, int nx, int ny, long long NX) {
+  __device__ void init(int tx, int ty, int x0, int y0, int nx, int ny) {
     const int xw = wrap(x0 + tx, nx), yw = wrap(y0 + ty, ny);
     hx = tx < RR || tx >= 64 - RR;
     ht = ty < RR;
     hb = ty >= TY - RR;
     lcx = tx < RR ? tx : tx + 2 * RR;
-    gx = (tx < RR ? wrap(x0 - RR + tx, nx) : wrap(x0 + tx + RR, nx)) + NX * yw;
-    gt = xw + NX * wrap(y0 - RR + ty, ny);
-    gb = xw + NX * wrap(y0 + ty + RR, ny);
+    gx = (unsigned)((tx < RR ? wrap(x0 - RR + tx, nx) : wrap(x0 + tx + RR, nx)) + nx * yw) * (unsigned)sizeof(R);
+    gt = (unsigned)(xw + nx * wrap(y0 - RR + ty, ny)) * (unsigned)sizeof(R);
+    gb = (unsigned)(xw + nx * wrap(y0 + ty + RR, ny)) * (unsigned)sizeof(R);
     if (ONE_Y) {
       lry = ht ? ty : ty + 2 * RR;
       if (hb) gt = gb;
       ht = ht || hb;
     }
   }
-  __device__ void load(const R* f, long long S, long long pz) {
+  // halo of the plane starting at byte offset pz
+  __device__ void load(const Comps<R, NC>& f, unsigned pz) {
     if (hx)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) vx[c] = f[c * S + gx + pz];
+      for (int c = 0; c < NC; ++c) vx[c] = ldo(f.c[c], gx + pz);
     if (ht)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) vt[c] = f[c * S + gt + pz];
+      for (int c = 0; c < NC; ++c) vt[c] = ldo(f.c[c], gt + pz);
     if constexpr (!ONE_Y) {
       if (hb)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) vb[c] = f[c * S + gb + pz];
+        for (int c = 0; c < NC; ++c) vb[c] = ldo(f.c[c], gb + pz);
     }
   }
   __device__ void store(R (*sm)[LY][LX], int tx, int ty) {
