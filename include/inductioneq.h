@@ -174,11 +174,13 @@ long long ieq_member_stride(int nx, int ny, int nz);
  * chosen by timing the candidates (member fields are saved and restored around it). */
 int ieq_ensemble_create(ieq_ensemble **ens, ieq_program *prog, const ieq_member *members, int nmembers, const char *integrator);
 
-/* Advance every member by nsteps steps of its own dt (hipGraph replay of one step, or two
- * for the 2N schemes).  b1 of every member holds the state on return. */
+/* Advance every member by nsteps steps of its own dt, in periods of one step (two for the
+ * 2N schemes, which must get an even nsteps).  b1 of every member holds the state on return. */
 int ieq_ensemble_step(ieq_ensemble *ens, long long nsteps, double *kernel_seconds);
 
-/* Chosen launch configuration: tile rows, z-chunk, order (0 member order, 1 largest first). */
+/* Chosen launch configuration: tile rows, z-chunk, order (0 member order, 1 largest first;
+ * +2 when the step is replayed from a hipGraph rather than launched directly, whichever
+ * timed faster). */
 int ieq_ensemble_config(const ieq_ensemble *ens, int *tile_rows, int *z_chunk, int *order);
 
 void ieq_ensemble_release(ieq_ensemble *ens);

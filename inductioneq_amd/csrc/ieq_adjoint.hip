@@ -47,50 +47,41 @@ template <typename R, int RR, int TY>
 __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
   constexpr int Q = 2 * RR + 1;
   constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR;
-  __shared__ R sY[3][LY][LX];
-  __shared__ R sL[3][LY][LX];
+  __shared__ R sYb[2][3][LY][LX];  // double-buffered planes: plane z in buffer (z - z0) & 1
+  __shared__ R sLb[2][3][LY][LX];
   __shared__ double red[4][64 * TY / 64];
   const int tx = threadIdx.x, ty = threadIdx.y;
   int x0, y0, z0;
   tile_origin<TY>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
   const bool active = x0 + tx < a.nx && y0 + ty < a.ny;
-  const long long S = a.S, NX = a.nx, NXY = (long long)a.nx * a.ny;
-  const long long col = wrap(x0 + tx, a.nx) + NX * wrap(y0 + ty, a.ny);
+  const long long S = a.S;
+  const unsigned RB = sizeof(R);
+  const unsigned colb = (unsigned)(wrap(x0 + tx, a.nx) + a.nx * wrap(y0 + ty, a.ny)) * RB;
+  const unsigned plb = (unsigned)(a.nx * a.ny) * RB;
   const int zend = min(z0 + a.zc, a.nz);
   PlaneStage<R, RR, TY, 3> hy, hl;
   hy.init(tx, ty, x0, y0, a.nx, a.ny);
   hl.init(tx, ty, x0, y0, a.nx, a.ny);
   const Comps<R, 3> Yc = comps<R, 3>(a.Y, S), Lc = comps<R, 3>(a.L, S);
-  const unsigned plb = (unsigned)(a.nx * a.ny) * (unsigned)sizeof(R);
 
-  R qy[3][Q], ql[2][Q];  // z-queues: Y (all components), L (x, y components)
+  R qy[3][Q + 1], ql[2][Q + 1];  // z-queues (planes z-RR .. z+RR, slot Q: plane z+RR+1 in flight)
+  R lz;                          // L_z of the own column at the plane staged next
 #pragma unroll
-  for (int j = 0; j < Q - 1; ++j) {
-    const long long id = col + NXY * wrap(z0 - RR + j, a.nz);
+  for (int j = 0; j < Q; ++j) {
+    const unsigned o = colb + (unsigned)wrap(z0 - RR + j, a.nz) * plb;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) qy[c][j + 1] = a.Y[c * S + id];
-    ql[0][j + 1] = a.L[id];
-    ql[1][j + 1] = a.L[S + id];
+    for (int c = 0; c < 3; ++c) qy[c][j] = ldo(Yc.c[c], o);
+    ql[0][j] = ldo(Lc.c[0], o);
+    ql[1][j] = ldo(Lc.c[1], o);
   }
+  hy.load(Yc, (unsigned)z0 * plb);
+  hl.load(Lc, (unsigned)z0 * plb);
+  lz = ldo(Lc.c[2], colb + (unsigned)z0 * plb);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int z = z0; z < zend; ++z) {
-#pragma unroll
-    for (int j = 0; j < Q - 1; ++j) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) qy[c][j] = qy[c][j + 1];
-      ql[0][j] = ql[0][j + 1];
-      ql[1][j] = ql[1][j + 1];
-    }
-    const long long idn = col + NXY * wrap(z + RR, a.nz);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) qy[c][Q - 1] = a.Y[c * S + idn];
-    ql[0][Q - 1] = a.L[idn];
-    ql[1][Q - 1] = a.L[S + idn];
-    const long long ic = col + NXY * z;
-    const R lz = a.L[2 * S + ic];
-    hy.load(Yc, (unsigned)z * plb);
-    hl.load(Lc, (unsigned)z * plb);
-    __syncthreads();  // previous plane's LDS reads are done
+    const int buf = (z - z0) & 1;
+    auto& sY = sYb[buf];
+    auto& sL = sLb[buf];
 #pragma unroll
     for (int c = 0; c < 3; ++c) sY[c][ty + RR][tx + RR] = qy[c][RR];
     sL[0][ty + RR][tx + RR] = ql[0][RR];
@@ -98,7 +89,19 @@ __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
     sL[2][ty + RR][tx + RR] = lz;
     hy.store(sY, tx, ty);
     hl.store(sL, tx, ty);
-    __syncthreads();
+    // next plane's loads, in the order they are consumed: halos and L_z, then the column
+    if (z + 1 < zend) {
+      hy.load(Yc, (unsigned)(z + 1) * plb);
+      hl.load(Lc, (unsigned)(z + 1) * plb);
+      lz = ldo(Lc.c[2], colb + (unsigned)(z + 1) * plb);
+      const unsigned o = colb + (unsigned)wrap(z + RR + 1, a.nz) * plb;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) qy[c][Q] = ldo(Yc.c[c], o);
+      ql[0][Q] = ldo(Lc.c[0], o);
+      ql[1][Q] = ldo(Lc.c[1], o);
+    }
+    const unsigned ic = colb + (unsigned)z * plb;
+    lds_barrier();
     if (active) {
       R v[Q];
       R dY[3][3];  // dY[d][c] = (D_d Y_c) / h_d
@@ -129,15 +132,15 @@ __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
       const R Mx = dyLz - dzLy, My = dzLx - dxLz, Mz = dxLy - dyLx;
       const R Jx = dY[1][2] - dY[2][1], Jy = dY[2][0] - dY[0][2], Jz = dY[0][1] - dY[1][0];
       const R Yx = qy[0][RR], Yy = qy[1][RR], Yz = qy[2][RR];
-      const R Lx = ql[0][RR], Ly = ql[1][RR], Lz = lz;
+      const R Lx = ql[0][RR], Ly = ql[1][RR], Lz = sL[2][ty + RR][tx + RR];
       if (a.hall) {
         const R ch = a.c_hall;
-        a.F[ic] = ch * (Yy * Mz - Yz * My);
-        a.F[S + ic] = ch * (Yz * Mx - Yx * Mz);
-        a.F[2 * S + ic] = ch * (Yx * My - Yy * Mx);
-        a.G[ic] = ch * (My * Jz - Mz * Jy);
-        a.G[S + ic] = ch * (Mz * Jx - Mx * Jz);
-        a.G[2 * S + ic] = ch * (Mx * Jy - My * Jx);
+        sto(a.F, ic, ch * (Yy * Mz - Yz * My));
+        sto(a.F + S, ic, ch * (Yz * Mx - Yx * Mz));
+        sto(a.F + 2 * S, ic, ch * (Yx * My - Yy * Mx));
+        sto(a.G, ic, ch * (My * Jz - Mz * Jy));
+        sto(a.G + S, ic, ch * (Mz * Jx - Mx * Jz));
+        sto(a.G + 2 * S, ic, ch * (Mx * Jy - My * Jx));
         // -<M, J x Y>
         const double jxy0 = (double)Jy * Yz - (double)Jz * Yy, jxy1 = (double)Jz * Yx - (double)Jx * Yz,
                      jxy2 = (double)Jx * Yy - (double)Jy * Yx;
@@ -148,6 +151,13 @@ __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
 #pragma unroll
       for (int k = 0; k < 3; ++k)
         acc[1 + k] += -(Ld[0] * dY[k][0] + Ld[1] * dY[k][1] + Ld[2] * dY[k][2]) + Ld[k] * divY;
+    }
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) qy[c][j] = qy[c][j + 1];
+      ql[0][j] = ql[0][j + 1];
+      ql[1][j] = ql[1][j + 1];
     }
   }
   // block reduction in a fixed order

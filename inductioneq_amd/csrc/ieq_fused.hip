@@ -13,11 +13,8 @@
 // update.  Both march along z with register queues of the z-stencil and stage one
 // xy-plane (plus halo) at a time in LDS for the x/y stencils.
 #include <cmath>
-#include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <type_traits>
-#include <utility>
 
 #include "ieq_device.hpp"
 #include "ieq_fused.hpp"
@@ -31,17 +28,10 @@ struct FusedWork {
   size_t scratch_bytes = 0;
   double* red = nullptr;    // uniformity check partials
   double* red_host = nullptr;
-  // the captured launches of one period (one step, or two for 2N schemes with an odd stage
-  // count) for the arguments in graph_key
-  hipStream_t cap = nullptr;
-  hipGraphExec_t exec = nullptr;
-  std::string graph_key;
 };
 
 void fused_release(FusedWork* w) {
   if (!w) return;
-  if (w->exec) (void)hipGraphExecDestroy(w->exec);
-  if (w->cap) (void)hipStreamDestroy(w->cap);
   if (w->scratch) (void)hipFree(w->scratch);
   if (w->red) (void)hipFree(w->red);
   if (w->red_host) (void)hipHostFree(w->red_host);
@@ -126,9 +116,6 @@ namespace {
 // complete J(it).  The column value and the halo of a plane are therefore read in the same
 // iteration, which is also when the neighbouring tiles (dispatched together to the same
 // XCD) read those rows as their own column values, so the halo rows are L2 hits.
-// Loads run two planes ahead into register sets chosen by the parity of the iteration
-// (the loop is unrolled by two), so no value is consumed or copied in the iteration that
-// issued its load.
 template <typename R, int RR, int TY, int NBUF = 2>
 __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int z0, R (*sB)[3][TY + 2 * RR][64 + 2 * RR]) {
   constexpr int Q = 2 * RR + 1;
@@ -139,11 +126,11 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
   const unsigned plb = (unsigned)(a.nx * a.ny) * RB;                                       // plane stride
   const int zend = min(z0 + a.zc, a.nz);
   const Comps<R, 3> B = comps<R, 3>(a.Y, a.S);
-  PlaneStage<R, RR, TY, 3, 2> hs;
+  PlaneStage<R, RR, TY, 3> hs;
   hs.init(tx, ty, x0, y0, a.nx, a.ny);
 
   R q[3][Q];       // own column: planes it-RR .. it+RR
-  R la[2][3];      // landing sets: column plane loaded at an iteration of parity s
+  R ql[3];         // landing register: plane it+RR+1
   R jr[3][RR + 1];  // x/y parts of curl_D B for planes it .. it+RR: (d_y B_z, -d_x B_z, d_x B_y - d_y B_x)
   const int it0 = z0 - RR;
 #pragma unroll
@@ -152,31 +139,24 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
 #pragma unroll
     for (int c = 0; c < 3; ++c) q[c][j] = ldo(B.c[c], o);
   }
-  {  // plane z0+1 enters the queue at the end of iteration it0
-    const unsigned o = colb + (unsigned)wrap(z0 + 1, a.nz) * plb;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) la[1][c] = ldo(B.c[c], o);
-  }
-  hs.template load<0>(B, (unsigned)z0 * plb);
-  if (z0 + 1 < zend) hs.template load<1>(B, (unsigned)(z0 + 1) * plb);
-  auto body = [&](auto parc, int it) {
-    constexpr int PAR = decltype(parc)::value;  // (it - it0) & 1
+  hs.load(B, (unsigned)z0 * plb);
+  for (int it = it0; it < zend; ++it) {
     const int p = it + RR;  // plane staged in this iteration
-    const int buf = NBUF == 2 ? PAR : 0;
+    const int buf = NBUF == 2 ? (it - it0) & 1 : 0;
     if constexpr (NBUF == 1) {
       if (it > it0) lds_barrier();
     }
     if (p < zend) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = q[c][Q - 1];
-      hs.template store<PAR>(sB[buf], tx, ty);
+      hs.store(sB[buf], tx, ty);
     }
-    // loads two planes ahead, halo first (the memory counter retires in issue order)
-    if (p + 2 < zend) hs.template load<PAR>(B, (unsigned)(p + 2) * plb);
-    if (it + 2 < zend) {
-      const unsigned o = colb + (unsigned)wrap(p + 2, a.nz) * plb;
+    // next iteration's loads, halo first (the memory counter retires in issue order)
+    if (p + 1 < zend) hs.load(B, (unsigned)(p + 1) * plb);
+    if (it + 1 < zend) {
+      const unsigned o = colb + (unsigned)wrap(p + 1, a.nz) * plb;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) la[PAR][c] = ldo(B.c[c], o);
+      for (int c = 0; c < 3; ++c) ql[c] = ldo(B.c[c], o);
     }
     lds_barrier();
     if (active) {
@@ -214,14 +194,10 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int j = 0; j < Q - 1; ++j) q[c][j] = q[c][j + 1];
-      q[c][Q - 1] = la[1 - PAR][c];  // plane it+RR+1, loaded one iteration earlier
+      q[c][Q - 1] = ql[c];
 #pragma unroll
       for (int k = 0; k < RR; ++k) jr[c][k] = jr[c][k + 1];
     }
-  };
-  for (int it = it0; it < zend; it += 2) {
-    body(std::integral_constant<int, 0>{}, it);
-    if (it + 1 < zend) body(std::integral_constant<int, 1>{}, it + 1);
   }
 }
 
@@ -278,8 +254,6 @@ __device__ __forceinline__ void load_point(const FArgs<R>& a, unsigned o, R* pv,
 // Stage kernel body, lead-aligned like emf_tile: iteration `it` stages plane p = it + RR of B
 // (and E) into LDS and evaluates the x/y-derivative terms of the RHS of plane p; the output
 // plane `it` adds the z-derivative terms from the register queues and applies the RK update.
-// Halo and column loads run two planes ahead into parity-selected register sets (loop
-// unrolled by two); the pointwise RK operands run one plane ahead.
 template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ = false, int NBUF = 2>
 __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, int z0, StageLds<R, RR, TY, HALL, NBUF>& L) {
   constexpr int Q = 2 * RR + 1;
@@ -294,16 +268,15 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
   const int zend = min(z0 + a.zc, a.nz);
   const Comps<R, 3> B = comps<R, 3>(a.Y, S);
   const Comps<R, 3> E = comps<R, 3>(HALL ? a.E : a.Y, S);
-  PlaneStage<R, RR, TY, 3, 2> hsb;
-  PlaneStage<R, RR, TY, 3, 2> hse;
+  PlaneStage<R, RR, TY, 3> hsb;
+  PlaneStage<R, RR, TY, 3> hse;
   hsb.init(tx, ty, x0, y0, a.nx, a.ny);
   if constexpr (HALL) hse.init(tx, ty, x0, y0, a.nx, a.ny);
 
-  R qb[3][Q], lb[2][3];                  // B of the own column: planes it-RR .. it+RR; landing sets
-  R qe[2][HALL ? Q : 1], le[2][2];       // E_x, E_y of the own column (z-derivatives of curl E)
-  R ez[2];                               // E_z of the own column for the plane staged at that parity
-  R rx[3][RR + 1];                       // x/y terms of the RHS for planes it .. it+RR
-  R pp[2][3], qq[2][3], gg[2][ADJ ? 3 : 1];  // pointwise RK operands by parity of the output plane
+  R qb[3][Q], qbl[3];                  // B of the own column: planes it-RR .. it+RR; landing it+RR+1
+  R qe[2][HALL ? Q : 1], qel[2];       // E_x, E_y of the own column (z-derivatives of curl E)
+  R ez = R(0);                         // E_z of the own column at the plane staged next
+  R rx[3][RR + 1];                     // x/y terms of the RHS for planes it .. it+RR
   const int it0 = z0 - RR;
 #pragma unroll
   for (int j = RR; j < Q; ++j) {  // planes z0-RR .. z0 (older slots are never read)
@@ -315,67 +288,60 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
       qe[1][j] = ldo(E.c[1], o);
     }
   }
-  {  // plane z0+1 enters the queues at the end of iteration it0
-    const unsigned o = colb + (unsigned)wrap(z0 + 1, a.nz) * plb;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) lb[1][c] = ldo(B.c[c], o);
-    if constexpr (HALL) {
-      le[1][0] = ldo(E.c[0], o);
-      le[1][1] = ldo(E.c[1], o);
-    }
-  }
-  hsb.template load<0>(B, (unsigned)z0 * plb);
+  hsb.load(B, (unsigned)z0 * plb);
   if constexpr (HALL) {
-    hse.template load<0>(E, (unsigned)z0 * plb);
-    ez[0] = ldo(E.c[2], colb + (unsigned)z0 * plb);
+    hse.load(E, (unsigned)z0 * plb);
+    ez = ldo(E.c[2], colb + (unsigned)z0 * plb);
   }
-  if (z0 + 1 < zend) {
-    hsb.template load<1>(B, (unsigned)(z0 + 1) * plb);
-    if constexpr (HALL) {
-      hse.template load<1>(E, (unsigned)(z0 + 1) * plb);
-      ez[1] = ldo(E.c[2], colb + (unsigned)(z0 + 1) * plb);
-    }
-  }
-  // the output plane z0 is produced at an iteration of parity RR & 1
-  if (active) load_point<ADJ>(a, colb + (unsigned)z0 * plb, pp[RR & 1], qq[RR & 1], gg[RR & 1]);
-
-  auto body = [&](auto parc, int it) {
-    constexpr int PAR = decltype(parc)::value;  // (it - it0) & 1
+  // pointwise RK operands of the output plane (pv, qv, gv) and of the next one (pn, qn, gn)
+  R pv[3], qv[3], gv[ADJ ? 3 : 1], pn[3], qn[3], gn[ADJ ? 3 : 1];
+  if (active) load_point<ADJ>(a, colb + (unsigned)z0 * plb, pn, qn, gn);
+  for (int it = it0; it < zend; ++it) {
     const int p = it + RR;
-    const int buf = NBUF == 2 ? PAR : 0;
+    const int buf = NBUF == 2 ? (it - it0) & 1 : 0;
     if constexpr (NBUF == 1) {
       if (it > it0) lds_barrier();
     }
     if (p < zend) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) sB[buf][c][ty + RR][tx + RR] = qb[c][Q - 1];
-      hsb.template store<PAR>(sB[buf], tx, ty);
+      hsb.store(sB[buf], tx, ty);
       if constexpr (HALL) {
         sE[buf][0][ty + RR][tx + RR] = qe[0][Q - 1];
         sE[buf][1][ty + RR][tx + RR] = qe[1][Q - 1];
-        sE[buf][2][ty + RR][tx + RR] = ez[PAR];
-        hse.template store<PAR>(sE[buf], tx, ty);
+        sE[buf][2][ty + RR][tx + RR] = ez;
+        hse.store(sE[buf], tx, ty);
       }
     }
-    // loads in the order they are consumed (the memory counter retires in issue order and
-    // also counts stores): halo of plane p+2, pointwise operands of output plane it+1,
-    // column plane p+2
-    if (p + 2 < zend) {
-      hsb.template load<PAR>(B, (unsigned)(p + 2) * plb);
-      if constexpr (HALL) {
-        hse.template load<PAR>(E, (unsigned)(p + 2) * plb);
-        ez[PAR] = ldo(E.c[2], colb + (unsigned)(p + 2) * plb);
-      }
-    }
-    if (it + 1 > z0 && it + 1 < zend && active)
-      load_point<ADJ>(a, colb + (unsigned)(it + 1) * plb, pp[1 - PAR], qq[1 - PAR], gg[1 - PAR]);
-    if (it + 2 < zend) {
-      const unsigned o = colb + (unsigned)wrap(p + 2, a.nz) * plb;
+    if (it >= z0) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) lb[PAR][c] = ldo(B.c[c], o);
+      for (int c = 0; c < 3; ++c) {
+        pv[c] = pn[c];
+        qv[c] = qn[c];
+      }
+      if constexpr (ADJ) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gv[c] = gn[c];
+      }
+    }
+    // next iteration's loads in the order they are consumed (the memory counter retires in
+    // issue order and also counts stores): halo of plane p+1, pointwise operands of output
+    // plane it+1, column plane p+1
+    if (p + 1 < zend) {
+      hsb.load(B, (unsigned)(p + 1) * plb);
       if constexpr (HALL) {
-        le[PAR][0] = ldo(E.c[0], o);
-        le[PAR][1] = ldo(E.c[1], o);
+        hse.load(E, (unsigned)(p + 1) * plb);
+        ez = ldo(E.c[2], colb + (unsigned)(p + 1) * plb);
+      }
+    }
+    if (it + 1 > z0 && it + 1 < zend && active) load_point<ADJ>(a, colb + (unsigned)(it + 1) * plb, pn, qn, gn);
+    if (it + 1 < zend) {
+      const unsigned o = colb + (unsigned)wrap(p + 1, a.nz) * plb;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) qbl[c] = ldo(B.c[c], o);
+      if constexpr (HALL) {
+        qel[0] = ldo(E.c[0], o);
+        qel[1] = ldo(E.c[1], o);
       }
     }
     lds_barrier();
@@ -463,11 +429,9 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
 #pragma unroll
           for (int c = 0; c < 3; ++c) rhs[c] = rhs[c] + hodsum<R, RR>(a.c, qb[c]);
         }
-        const R* pv = pp[PAR];
-        const R* qv = qq[PAR];
         if constexpr (ADJ) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c) rhs[c] = rhs[c] - gg[PAR][c];
+          for (int c = 0; c < 3; ++c) rhs[c] = rhs[c] - gv[c];
         }
         const unsigned oc = colb + (unsigned)it * plb;
         if (a.mode == 0) {
@@ -493,7 +457,7 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int j = 0; j < Q - 1; ++j) qb[c][j] = qb[c][j + 1];
-      qb[c][Q - 1] = lb[1 - PAR][c];  // plane it+RR+1, loaded one iteration earlier
+      qb[c][Q - 1] = qbl[c];
 #pragma unroll
       for (int k = 0; k < RR; ++k) rx[c][k] = rx[c][k + 1];
     }
@@ -502,13 +466,9 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
       for (int c = 0; c < 2; ++c) {
 #pragma unroll
         for (int j = 0; j < Q - 1; ++j) qe[c][j] = qe[c][j + 1];
-        qe[c][Q - 1] = le[1 - PAR][c];
+        qe[c][Q - 1] = qel[c];
       }
     }
-  };
-  for (int it = it0; it < zend; it += 2) {
-    body(std::integral_constant<int, 0>{}, it);
-    if (it + 1 < zend) body(std::integral_constant<int, 1>{}, it + 1);
   }
 }
 
@@ -847,8 +807,6 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
   a.zc = env_int("IEQ_FUSED_ZC", 8);
   if (a.zc < 1) a.zc = 1;
   const int ty = env_int("IEQ_FUSED_TY", 8) == 4 ? 4 : 8;
-  // EMF tiles: 4 rows (256 threads) let three blocks share a CU at the EMF kernel's register use
-  const int ty_emf = env_int("IEQ_FUSED_EMF_TY", 4) == 8 ? 8 : 4;
   a.S = info.stride;
   a.c = coef;
   const double h[3] = {p.dx, p.dy, p.dz};
@@ -868,15 +826,14 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
 
   R* b1 = F[0];
   R* b1alt = nf == 2 ? (R*)w->scratch : nullptr;
-  // enqueue one step on stream q (b1 and b1alt swap at every 2N stage)
-  auto enqueue_step = [&](hipStream_t q) -> hipError_t {
+  for (long long step = 0; step < nsteps; ++step) {
     for (const StageSpec& st : plan) {
       a.P = nullptr;
       a.Q = nullptr;
       if (st.kind == 3) {  // EMF of the field the next stage differentiates
         a.Y = st.Y == 0 ? b1 : F[st.Y];
         a.Eout = J;
-        emf<R>(rr, ty_emf, a, q);
+        emf<R>(rr, ty, a, s);
       } else if (st.kind == 0) {
         R* FF[3] = {b1, F[1], F[2]};
         a.Y = FF[st.Y];
@@ -888,14 +845,14 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
         a.cP = (R)st.cP;
         a.cQ = (R)st.cQ;
         a.cD = (R)(st.cD * p.dt);
-        stage<R>(rr, ty, a, p.hall, hod, q);
+        stage<R>(rr, ty, a, p.hall, hod, s);
       } else if (st.kind == 1) {
         uint32_t g[3] = {(uint32_t)p.nx, (uint32_t)p.ny, (uint32_t)p.nz};
         R* FF[3] = {b1, F[1], F[2]};
-        gen_update_2r<R>(p, q, g, FF[st.P], FF[st.out], FF[2], (R)st.cP, (R)st.cQ, time, -1.0);
+        gen_update_2r<R>(p, s, g, FF[st.P], FF[st.out], FF[2], (R)st.cP, (R)st.cQ, time, -1.0);
       } else if (st.kind == 2) {
         uint32_t g[3] = {(uint32_t)p.nx, (uint32_t)p.ny, (uint32_t)p.nz};
-        gen_ssprk104_06<R>(p, q, g, b1, F[1], F[2], time);
+        gen_ssprk104_06<R>(p, s, g, b1, F[1], F[2], time);
       } else {  // 2N: b2 = A*b2 + DT*RHS(b1); b1' = b1 + B*b2 into the other buffer
         a.Y = b1;
         a.E = J;
@@ -905,52 +862,14 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
         a.cP = (R)st.cP;
         a.cQ = (R)st.cQ;
         a.cD = DT;
-        stage<R>(rr, ty, a, p.hall, hod, q);
+        stage<R>(rr, ty, a, p.hall, hod, s);
         R* t = b1;
         b1 = b1alt;
         b1alt = t;
       }
       hipError_t e = hipGetLastError();
-      if (e != hipSuccess) return e;
+      if (e != hipSuccess) return err = std::string("fused launch: ") + hipGetErrorString(e), IEQ_ERR_HIP;
     }
-    return hipSuccess;
-  };
-  int n2n = 0;
-  for (const StageSpec& st : plan) n2n += st.kind == 4;
-  const int period = (n2n % 2) ? 2 : 1;  // steps after which b1 is back in its own buffer
-  long long done = 0;
-  if (nsteps >= 2 * period) {
-    // replay the launches of one period from a hipGraph (captured once per argument set)
-    char key[512];
-    std::snprintf(key, sizeof(key), "%s|%p|%p|%p|%p|%p|%p|%d|%d|%d|%.17g|%.17g|%.17g|%.17g|%.17g|%.17g|%d|%d",
-                  integrator, (void*)F[0], (void*)F[1], (void*)F[2], (void*)J, (void*)b1alt, (void*)time, rr, ty * 16 + ty_emf, a.zc,
-                  p.dt, (double)a.rho, (double)a.u[0], (double)a.u[1], (double)a.u[2], p.ho_factor, a.conv, (int)sizeof(R));
-    if (w->graph_key != key || !w->exec) {
-      if (w->exec) (void)hipGraphExecDestroy(w->exec);
-      w->exec = nullptr;
-      w->graph_key.clear();
-      if (!w->cap && hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking) != hipSuccess)
-        return err = "stream creation failed", IEQ_ERR_HIP;
-      hipGraph_t g = nullptr;
-      if (hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
-        return err = "graph capture failed", IEQ_ERR_HIP;
-      hipError_t e = hipSuccess;
-      for (int k = 0; k < period && e == hipSuccess; ++k) e = enqueue_step(w->cap);
-      const hipError_t ec = hipStreamEndCapture(w->cap, &g);
-      if (e != hipSuccess || ec != hipSuccess) return err = "graph capture failed", IEQ_ERR_HIP;
-      const hipError_t ei = hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (ei != hipSuccess) return err = "graph instantiation failed", IEQ_ERR_HIP;
-      w->graph_key = key;
-    }
-    b1 = F[0];  // the capture swapped the 2N registers an even number of times
-    b1alt = nf == 2 ? (R*)w->scratch : nullptr;
-    for (; done + period <= nsteps; done += period)
-      if (hipGraphLaunch(w->exec, s) != hipSuccess) return err = "graph launch failed", IEQ_ERR_HIP;
-  }
-  for (; done < nsteps; ++done) {
-    hipError_t e = enqueue_step(s);
-    if (e != hipSuccess) return err = std::string("fused launch: ") + hipGetErrorString(e), IEQ_ERR_HIP;
   }
   if (b1 != F[0]) {
     if (hipMemcpyAsync(F[0], b1, vbytes, hipMemcpyDeviceToDevice, s) != hipSuccess) return err = "copy-back failed", IEQ_ERR_HIP;
@@ -983,6 +902,7 @@ struct Ensemble {
   int4* dtiles = nullptr;
   int ntiles = 0, ty = 8, zc = 8, order = 0, grid_emf = 0, grid_stage = 0;
   bool tuned = false;
+  bool use_graph = true;  // replay the captured step, or launch directly (chosen by timing)
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1227,11 +1147,42 @@ template <typename R> int ens_tune(Ensemble& e, std::string& err) {
     }
   }
   (void)hipStreamSynchronize(e.stream);
-  (void)hipFree(save);
   int rc = ens_configure<R>(e, bty, bzc, bord, err);
   if (rc) return rc;
+  rc = ens_capture<R>(e, err);
+  if (rc) return rc;
+  {  // graph replay against direct launches of the same period (fields restored afterwards)
+    hipEvent_t u0, u1, u2;
+    (void)hipEventCreate(&u0);
+    (void)hipEventCreate(&u1);
+    (void)hipEventCreate(&u2);
+    off = 0;
+    (void)hipGraphLaunch(e.exec, e.stream);  // warm-up
+    (void)hipEventRecord(u0, e.stream);
+    for (int k = 0; k < 2; ++k) (void)hipGraphLaunch(e.exec, e.stream);
+    (void)hipEventRecord(u1, e.stream);
+    for (int k = 0; k < 2; ++k) (void)ens_launch_steps<R>(e, e.stream);
+    (void)hipEventRecord(u2, e.stream);
+    (void)hipEventSynchronize(u2);
+    float tg = 0.f, td = 0.f;
+    (void)hipEventElapsedTime(&tg, u0, u1);
+    (void)hipEventElapsedTime(&td, u1, u2);
+    e.use_graph = tg <= td;
+    (void)hipEventDestroy(u0);
+    (void)hipEventDestroy(u1);
+    (void)hipEventDestroy(u2);
+    for (auto& mb : e.m) {
+      void* f[3] = {mb.b1, mb.b2, mb.b3};
+      for (int j = 0; j < 3; ++j) {
+        (void)hipMemcpyAsync(f[j], save + off, member_bytes(e, mb), hipMemcpyDeviceToDevice, e.stream);
+        off += member_bytes(e, mb);
+      }
+    }
+    (void)hipStreamSynchronize(e.stream);
+  }
+  (void)hipFree(save);
   e.tuned = true;
-  return ens_capture<R>(e, err);
+  return hipGetLastError() == hipSuccess ? 0 : (err = "ensemble tuning failed", -4);
 }
 
 }  // namespace
@@ -1295,8 +1246,14 @@ int ensemble_step(Ensemble* e, long long nsteps, double* seconds, std::string& e
   (void)hipEventCreate(&t0);
   (void)hipEventCreate(&t1);
   (void)hipEventRecord(t0, e->stream);
-  for (long long k = 0; k < nsteps; k += e->steps_per_graph)
-    if (hipGraphLaunch(e->exec, e->stream) != hipSuccess) return err = "graph launch failed", IEQ_ERR_HIP;
+  for (long long k = 0; k < nsteps; k += e->steps_per_graph) {
+    if (e->use_graph) {
+      if (hipGraphLaunch(e->exec, e->stream) != hipSuccess) return err = "graph launch failed", IEQ_ERR_HIP;
+    } else {
+      hipError_t r = e->real_bytes == 8 ? ens_launch_steps<double>(*e, e->stream) : ens_launch_steps<float>(*e, e->stream);
+      if (r != hipSuccess) return err = std::string("ensemble launch failed: ") + hipGetErrorString(r), IEQ_ERR_HIP;
+    }
+  }
   (void)hipEventRecord(t1, e->stream);
   if (hipEventSynchronize(t1) != hipSuccess) return err = "ensemble step failed", IEQ_ERR_HIP;
   float ms = 0.f;
@@ -1310,7 +1267,7 @@ int ensemble_step(Ensemble* e, long long nsteps, double* seconds, std::string& e
 void ensemble_config(const Ensemble* e, int* ty, int* zc, int* order) {
   *ty = e->ty;
   *zc = e->zc;
-  *order = e->order;
+  *order = e->order + (e->use_graph ? 2 : 0);
 }
 
 void ensemble_release(Ensemble* e) {

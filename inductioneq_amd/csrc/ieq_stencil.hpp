@@ -19,6 +19,7 @@ template <typename R> struct StencilCoef {
 
 template <typename R> struct FArgs {
   int nx, ny, nz, zc;
+  int nbuf;  // LDS plane buffers of the single-grid kernels: 2 (pipelined) or 1 (half the LDS)
   long long S;
   StencilCoef<R> c;
   R inv_h[3];  // (R)(1/dx_d): Hall_volume scaling
@@ -121,16 +122,14 @@ template <typename R, int NC> __device__ __forceinline__ Comps<R, NC> comps(cons
 // Per-thread staging of one xy-plane of an NC-component field into an LDS tile with a
 // halo of RR cells.  The thread owns its column (x0+tx, y0+ty) and, at the tile edges,
 // up to one x-halo cell and up to two y-halo cells (all coordinates wrap periodically).
-// NSET value sets let a caller keep the halos of NSET planes in flight (set chosen at
-// compile time).
-template <typename R, int RR, int TY, int NC, int NSET = 1> struct PlaneStage {
+template <typename R, int RR, int TY, int NC> struct PlaneStage {
   static constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR;
   // With TY >= 2*RR a thread stages at most one y-halo cell (top rows or bottom rows).
   static constexpr bool ONE_Y = TY >= 2 * RR;
   bool hx, ht, hb;
   int lcx, lry;                // LDS column of the x-halo cell; LDS row of the y-halo cell (ONE_Y)
   unsigned gx, gt, gb;         // in-plane byte offsets of the halo cells
-  R vx[NSET][NC], vt[NSET][NC], vb[NSET][ONE_Y ? 1 : NC];  // prefetched halo values
+  R vx[NC], vt[NC], vb[ONE_Y ? 1 : NC];  // prefetched halo values of the next plane to stage
 
   __device__ void init(int tx, int ty, int x0, int y0, int nx, int ny) {
     const int xw = wrap(x0 + tx, nx), yw = wrap(y0 + ty, ny);
@@ -147,35 +146,35 @@ template <typename R, int RR, int TY, int NC, int NSET = 1> struct PlaneStage {
       ht = ht || hb;
     }
   }
-  // halo of the plane starting at byte offset pz, into value set SET
-  template <int SET = 0> __device__ void load(const Comps<R, NC>& f, unsigned pz) {
+  // halo of the plane starting at byte offset pz
+  __device__ void load(const Comps<R, NC>& f, unsigned pz) {
     if (hx)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) vx[SET][c] = ldo(f.c[c], gx + pz);
+      for (int c = 0; c < NC; ++c) vx[c] = ldo(f.c[c], gx + pz);
     if (ht)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) vt[SET][c] = ldo(f.c[c], gt + pz);
+      for (int c = 0; c < NC; ++c) vt[c] = ldo(f.c[c], gt + pz);
     if constexpr (!ONE_Y) {
       if (hb)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) vb[SET][c] = ldo(f.c[c], gb + pz);
+        for (int c = 0; c < NC; ++c) vb[c] = ldo(f.c[c], gb + pz);
     }
   }
-  template <int SET = 0> __device__ void store(R (*sm)[LY][LX], int tx, int ty) {
+  __device__ void store(R (*sm)[LY][LX], int tx, int ty) {
     if (hx)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) sm[c][ty + RR][lcx] = vx[SET][c];
+      for (int c = 0; c < NC; ++c) sm[c][ty + RR][lcx] = vx[c];
     if constexpr (ONE_Y) {
       if (ht)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) sm[c][lry][tx + RR] = vt[SET][c];
+        for (int c = 0; c < NC; ++c) sm[c][lry][tx + RR] = vt[c];
     } else {
       if (ht)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) sm[c][ty][tx + RR] = vt[SET][c];
+        for (int c = 0; c < NC; ++c) sm[c][ty][tx + RR] = vt[c];
       if (hb)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) sm[c][ty + 2 * RR][tx + RR] = vb[SET][c];
+        for (int c = 0; c < NC; ++c) sm[c][ty + 2 * RR][tx + RR] = vb[c];
     }
   }
 };

@@ -3,9 +3,9 @@
 Every member is a periodic box with its own grid, spacing, time step, uniform density
 and uniform velocity; all members share one program's settings (REAL type, operator,
 Hall term, dissipation, flux forms).  Each RK stage of the whole ensemble is one
-persistent launch per fused kernel over a tile list that covers every member, and one
-step (two for the 2N schemes) is replayed from a captured hipGraph.  The tile shape,
-z-chunk and tile order are chosen by timing on the first step.
+persistent launch per fused kernel over a tile list that covers every member.  The tile
+shape, z-chunk and tile order, and whether one step (two for the 2N schemes) is replayed
+from a captured hipGraph or launched directly, are chosen by timing on the first step.
 
 There is no counterpart in the reference, which builds one OpenCL program per solve
 (``matlab/+induction_eq/compute_numerical_solution.m:23``); each member's arithmetic is
@@ -94,4 +94,5 @@ class Ensemble:
         ty, zc, order = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _capi.check(self.lib.ieq_ensemble_config(self.handle, ctypes.byref(ty), ctypes.byref(zc), ctypes.byref(order)),
                     self.program.handle)
-        return {"tile_rows": ty.value, "z_chunk": zc.value, "order": ("member", "largest_first")[order.value]}
+        return {"tile_rows": ty.value, "z_chunk": zc.value, "order": ("member", "largest_first")[order.value & 1],
+                "graph": bool(order.value & 2)}
