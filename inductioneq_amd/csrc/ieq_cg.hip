@@ -3,10 +3,10 @@
 // Restates the host loop of matlab/+divergence_cleaning/clean_div_CG.m:31-86
 // (calc_div_divcleaning -> r; repeat { u = beta u + r; q = -Lap u; alpha = rr/(u.q);
 // phi += alpha u; r -= alpha q; rr = r.r } while rr > thr^2 and it < max; projection)
-// as three kernels per iteration with all scalars kept on the device:
-//   k_cg_dir     u = beta*u + r                    (beta = rr/old, old = 1 at start)
-//   k_cg_lap     q = -Lap u on the BNODES interior, partial u.q
-//   k_cg_update  phi += alpha u, r -= alpha q, partial r.r
+// as two kernels per iteration with all scalars kept on the device:
+//   k_cg_lap     u' = beta*u + r (beta = rr/old, old = 1 at start), q = -Lap u' on the
+//                BNODES interior, partial u'.q  (u' goes to the other direction register)
+//   k_cg_update  phi += alpha u', r -= alpha q, partial r.r
 // Dot products accumulate in FP64 whatever the field precision: every block writes
 // its partial, the last block to finish (atomic ticket) sums all partials in index
 // order, so the result is deterministic.  Once the stop test holds, every kernel
@@ -68,14 +68,6 @@ __device__ inline double sum_partials(const double* part, unsigned int n, double
   return block_sum(v, smem);
 }
 
-template <typename R>
-__global__ __launch_bounds__(kBlk) void k_cg_dir(R* u, const R* r, long long n, const CgState* st) {
-  if (st->done) return;
-  const R beta = (R)(st->rr / st->old);
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    u[i] = beta * u[i] + r[i];
-}
-
 struct LapArgs {
   int nx, ny, nz, off;  // grid and BNODES
   int zc;               // planes per block
@@ -85,22 +77,35 @@ struct LapArgs {
   unsigned int nblocks;
 };
 
-// q = -Lap u on [off, n-off)^3; neighbours outside the grid read 0 (their coefficients are 0).
+// Fused direction update and Laplacian on [off, n-off)^3:
+//   u' = beta u + r (beta = rr/old),  q = -Lap u',  partial u'.q
+// u' is written to a second buffer (neighbouring blocks still read u), so the direction
+// registers alternate between iterations.  Outside the interior u and r are 0 (never
+// written), so u' is 0 there as in the reference, where calc_axpy runs on the interior
+// only; neighbours outside the grid read 0 (their coefficients are 0).  Planes are
+// double-buffered in LDS and the loads of plane z+1 are in flight while plane z is computed.
 template <typename R, int HW, int TY>
-__global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* u, R* q, CgState* st, double* part) {
+__global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restrict__ u, const R* __restrict__ r,
+                                                    R* __restrict__ un, R* __restrict__ q, CgState* st, double* part) {
   constexpr int LX = 64 + 2 * HW, LY = TY + 2 * HW, Q = 2 * HW + 1;
-  __shared__ R s[LY][LX];
+  __shared__ R s[2][LY][LX];
   __shared__ double red[16];
   __shared__ bool last;
   if (st->done) return;
+  const R beta = (R)(st->rr / st->old);
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int gx = a.nx - 2 * a.off, gy = a.ny - 2 * a.off, gz = a.nz - 2 * a.off;
   const int x = a.off + blockIdx.x * 64 + tx, y = a.off + blockIdx.y * TY + ty;
   const int x0 = a.off + blockIdx.x * 64, y0 = a.off + blockIdx.y * TY;
   const bool active = (x - a.off) < gx && (y - a.off) < gy;
   const long long NX = a.nx, NXY = (long long)a.nx * a.ny;
-  auto val = [&](int xx, int yy, int zz) -> R {
-    return (xx >= 0 && xx < a.nx && yy >= 0 && yy < a.ny && zz >= 0 && zz < a.nz) ? u[xx + NX * yy + NXY * zz] : R(0);
+  // u' at a node (0 outside the grid)
+  auto dirv = [&](int xx, int yy, int zz) -> R {
+    if (xx >= 0 && xx < a.nx && yy >= 0 && yy < a.ny && zz >= 0 && zz < a.nz) {
+      const long long id = xx + NX * yy + NXY * zz;
+      return beta * u[id] + r[id];
+    }
+    return R(0);
   };
   // per-thread x/y rows
   R cx[Q], cy[Q];
@@ -116,18 +121,29 @@ __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* u, R* q,
   }
   double dot = 0.0;
   R qz[Q + 1];
+  R h[4];  // halo values of the plane being staged: x left, x right, y below, y above
   const int z0 = a.off + blockIdx.z * a.zc;
   const int zn = min(a.zc, gz - (int)blockIdx.z * a.zc);
+  auto load_halo = [&](int z) {
+    if (tx < HW) h[0] = dirv(x0 - HW + tx, y, z);
+    if (tx >= 64 - HW) h[1] = dirv(x + HW, y, z);
+    if (ty < HW) h[2] = dirv(x, y0 - HW + ty, z);
+    if (ty >= TY - HW) h[3] = dirv(x, y + HW, z);
+  };
 #pragma unroll
-  for (int j = 0; j < Q; ++j) qz[j] = val(x, y, z0 - HW + j);
+  for (int j = 0; j < Q; ++j) qz[j] = dirv(x, y, z0 - HW + j);
+  load_halo(z0);
   for (int zi = 0; zi < zn; ++zi) {
     const int z = z0 + zi;
-    qz[Q] = val(x, y, z + HW + 1);
-    s[ty + HW][tx + HW] = qz[HW];
-    if (tx < HW) s[ty + HW][tx] = val(x0 - HW + tx, y, z);
-    if (tx >= 64 - HW) s[ty + HW][tx + 2 * HW] = val(x + HW, y, z);
-    if (ty < HW) s[ty][tx + HW] = val(x, y0 - HW + ty, z);
-    if (ty >= TY - HW) s[ty + 2 * HW][tx + HW] = val(x, y + HW, z);
+    auto& sp = s[zi & 1];
+    sp[ty + HW][tx + HW] = qz[HW];
+    if (tx < HW) sp[ty + HW][tx] = h[0];
+    if (tx >= 64 - HW) sp[ty + HW][tx + 2 * HW] = h[1];
+    if (ty < HW) sp[ty][tx + HW] = h[2];
+    if (ty >= TY - HW) sp[ty + 2 * HW][tx + HW] = h[3];
+    // loads of the next plane (halo first), in flight across the barrier and the compute
+    if (zi + 1 < zn) load_halo(z + 1);
+    qz[Q] = dirv(x, y, z + HW + 1);
     __syncthreads();
     if (active) {
       const int rz = bound_row((uint32_t)z, (uint32_t)a.nz, a.nbl, 0);
@@ -137,13 +153,13 @@ __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* u, R* q,
       for (int k = 0; k < Q; ++k) {
         const int j = c0 - HW + k;
         const R cz = (j >= 0 && j < a.wl) ? (R)(a.tab[rz * a.wl + j] * a.idz2) : R(0);
-        acc = acc + cx[k] * s[ty + HW][tx + k] + cy[k] * s[ty + k][tx + HW] + cz * qz[k];
+        acc = acc + cx[k] * sp[ty + HW][tx + k] + cy[k] * sp[ty + k][tx + HW] + cz * qz[k];
       }
       const long long id = x + NX * y + NXY * z;
+      un[id] = qz[HW];
       q[id] = acc;
       dot += (double)qz[HW] * (double)acc;
     }
-    __syncthreads();
 #pragma unroll
     for (int j = 0; j < Q; ++j) qz[j] = qz[j + 1];
   }
@@ -217,10 +233,11 @@ __global__ __launch_bounds__(kBlk) void k_cg_init(const R* r, long long n, CgSta
   }
 }
 
-template <typename R, int HW, int TY> void launch_lap(const LapArgs& a, const R* u, R* q, CgState* st, double* part, hipStream_t s) {
+template <typename R, int HW, int TY>
+void launch_lap(const LapArgs& a, const R* u, const R* r, R* un, R* q, CgState* st, double* part, hipStream_t s) {
   const int gx = a.nx - 2 * a.off, gy = a.ny - 2 * a.off, gz = a.nz - 2 * a.off;
   dim3 grid((gx + 63) / 64, (gy + TY - 1) / TY, (gz + a.zc - 1) / a.zc);
-  k_cg_lap<R, HW, TY><<<grid, dim3(64, TY), 0, s>>>(a, u, q, st, part);
+  k_cg_lap<R, HW, TY><<<grid, dim3(64, TY), 0, s>>>(a, u, r, un, q, st, part);
 }
 
 }  // namespace
@@ -250,7 +267,7 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
   const size_t fbytes = ((size_t)info.stride * sizeof(R) + 255) / 256 * 256;
   const size_t part_bytes = (size_t)std::max(lap_blocks, vec_blocks) * sizeof(double);
   const size_t tab_bytes = (size_t)(2 * nbl + 1) * wl * sizeof(double);
-  const size_t need = 3 * fbytes + ((part_bytes + 255) / 256 * 256) + 256 + tab_bytes;
+  const size_t need = 4 * fbytes + ((part_bytes + 255) / 256 * 256) + 256 + tab_bytes;
   if (!*work) *work = new CgWork();
   CgWork* w = *work;
   if (w->bytes < need) {
@@ -262,12 +279,12 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
   }
   char* base = (char*)w->buf;
   R* r = (R*)base;
-  R* u = (R*)(base + fbytes);
+  R* ub[2] = {(R*)(base + fbytes), (R*)(base + 3 * fbytes)};  // direction registers, alternating
   R* q = (R*)(base + 2 * fbytes);
-  double* part = (double*)(base + 3 * fbytes);
-  CgState* st = (CgState*)(base + 3 * fbytes + (part_bytes + 255) / 256 * 256);
+  double* part = (double*)(base + 4 * fbytes);
+  CgState* st = (CgState*)(base + 4 * fbytes + (part_bytes + 255) / 256 * 256);
   double* tab = (double*)((char*)st + 256);
-  if (hipMemsetAsync(base, 0, 3 * fbytes, s) != hipSuccess || hipMemsetAsync(phi, 0, fbytes, s) != hipSuccess ||
+  if (hipMemsetAsync(base, 0, 4 * fbytes, s) != hipSuccess || hipMemsetAsync(phi, 0, fbytes, s) != hipSuccess ||
       hipMemsetAsync(st, 0, sizeof(CgState), s) != hipSuccess ||
       hipMemcpyAsync(tab, host_tab, tab_bytes, hipMemcpyHostToDevice, s) != hipSuccess)
     return err = "CG set-up failed", -4;
@@ -300,17 +317,19 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
   while (launched < max_it) {
     const int k = std::min(chunk, max_it - launched);
     for (int i = 0; i < k; ++i) {
-      k_cg_dir<R><<<vec_blocks, kBlk, 0, s>>>(u, r, n, st);
+      const int it = launched + i;
+      const R* u = ub[it & 1];
+      R* un = ub[(it + 1) & 1];
       switch (hw) {
-        case 1: launch_lap<R, 1, 8>(la, u, q, st, part, s); break;
-        case 2: launch_lap<R, 2, 8>(la, u, q, st, part, s); break;
-        case 3: launch_lap<R, 3, 8>(la, u, q, st, part, s); break;
-        case 4: launch_lap<R, 4, 8>(la, u, q, st, part, s); break;
-        case 5: launch_lap<R, 5, 8>(la, u, q, st, part, s); break;
-        case 6: launch_lap<R, 6, 8>(la, u, q, st, part, s); break;
-        default: launch_lap<R, 7, 8>(la, u, q, st, part, s); break;
+        case 1: launch_lap<R, 1, 8>(la, u, r, un, q, st, part, s); break;
+        case 2: launch_lap<R, 2, 8>(la, u, r, un, q, st, part, s); break;
+        case 3: launch_lap<R, 3, 8>(la, u, r, un, q, st, part, s); break;
+        case 4: launch_lap<R, 4, 8>(la, u, r, un, q, st, part, s); break;
+        case 5: launch_lap<R, 5, 8>(la, u, r, un, q, st, part, s); break;
+        case 6: launch_lap<R, 6, 8>(la, u, r, un, q, st, part, s); break;
+        default: launch_lap<R, 7, 8>(la, u, r, un, q, st, part, s); break;
       }
-      k_cg_update<R><<<vec_blocks, kBlk, 0, s>>>(phi, u, r, q, n, st, part, vec_blocks);
+      k_cg_update<R><<<vec_blocks, kBlk, 0, s>>>(phi, un, r, q, n, st, part, vec_blocks);
     }
     launched += k;
     if (hipMemcpyAsync(&hst, st, sizeof(CgState), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
