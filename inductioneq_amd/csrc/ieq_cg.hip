@@ -8,8 +8,8 @@
 //                BNODES interior, partial u'.q  (u' goes to the other direction register)
 //   k_cg_update  phi += alpha u', r -= alpha q, partial r.r
 // Dot products accumulate in FP64 whatever the field precision: every block writes
-// its partial, the last block to finish (atomic ticket) sums all partials in index
-// order, so the result is deterministic.  Once the stop test holds, every kernel
+// its partial and a one-workgroup kernel sums the partials in index order, so the
+// result is deterministic.  Once the stop test holds, every kernel
 // returns immediately, so any number of iterations can be enqueued without a host
 // round trip and the result equals stopping exactly at convergence.
 #include <cmath>
@@ -26,7 +26,6 @@ struct CgState {
   double tol2;       // stop when rr <= tol2 ...
   double rel2;       // ... or rr <= rel2 * rr0 (rel2 = 0: absolute test only)
   int it, max_it, done, pad;
-  unsigned int ticket_lap, ticket_upd;
 };
 
 namespace {
@@ -47,18 +46,6 @@ __device__ inline double block_sum(double v, double* smem) {
   }
   __syncthreads();
   return t;  // valid in wave 0
-}
-
-// Last-block finish: returns true in exactly one block, after all partials are visible.
-__device__ inline bool last_block(unsigned int* ticket, unsigned int nblocks, bool* flag_smem) {
-  __threadfence();
-  if (threadIdx.x == 0 && threadIdx.y == 0) {
-    unsigned int t = atomicAdd(ticket, 1u);
-    *flag_smem = (t == nblocks - 1);
-  }
-  __syncthreads();
-  if (*flag_smem) __threadfence();  // acquire: the other blocks' partials are visible
-  return *flag_smem;
 }
 
 __device__ inline double sum_partials(const double* part, unsigned int n, double* smem) {
@@ -90,7 +77,6 @@ __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restri
   constexpr int LX = 64 + 2 * HW, LY = TY + 2 * HW, Q = 2 * HW + 1;
   __shared__ R s[2][LY][LX];
   __shared__ double red[16];
-  __shared__ bool last;
   if (st->done) return;
   const R beta = (R)(st->rr / st->old);
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -166,20 +152,36 @@ __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restri
   const double bs = block_sum(dot, red);
   const unsigned int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   if (tx == 0 && ty == 0) part[bid] = bs;
-  if (last_block(&st->ticket_lap, a.nblocks, &last)) {
-    const double uq = sum_partials(part, a.nblocks, red);
-    if (tx == 0 && ty == 0) {
-      st->alpha = st->rr / uq;
-      st->ticket_lap = 0;
-    }
+}
+
+// One workgroup: alpha = rr / u'.q from the block partials (summed in block order).  A
+// separate launch instead of a last-block finish: a device-scope fence in every block
+// writes back and invalidates the L2 of its XCD.
+__global__ __launch_bounds__(256) void k_cg_alpha(const double* part, unsigned int n, CgState* st) {
+  __shared__ double red[16];
+  if (st->done) return;
+  const double uq = sum_partials(part, n, red);
+  if (threadIdx.x == 0) st->alpha = st->rr / uq;
+}
+
+// One workgroup: rr = r.r, the stop test and the iteration count.
+__global__ __launch_bounds__(256) void k_cg_rr(const double* part, unsigned int n, CgState* st) {
+  __shared__ double red[16];
+  if (st->done) return;
+  const double rr = sum_partials(part, n, red);
+  if (threadIdx.x == 0) {
+    st->old = st->rr;
+    st->rr = rr;
+    st->it += 1;
+    const bool go = rr > st->tol2 && (st->rel2 <= 0.0 || rr > st->rel2 * st->rr0) && st->it < st->max_it;
+    st->done = go ? 0 : 1;
   }
 }
 
 template <typename R>
-__global__ __launch_bounds__(kBlk) void k_cg_update(R* phi, const R* u, R* r, const R* q, long long n, CgState* st,
-                                                    double* part, unsigned int nblocks) {
+__global__ __launch_bounds__(kBlk) void k_cg_update(R* __restrict__ phi, const R* __restrict__ u, R* __restrict__ r,
+                                                    const R* __restrict__ q, long long n, CgState* st, double* part) {
   __shared__ double red[16];
-  __shared__ bool last;
   if (st->done) return;
   const R alpha = (R)st->alpha;
   double acc = 0.0;
@@ -191,45 +193,34 @@ __global__ __launch_bounds__(kBlk) void k_cg_update(R* phi, const R* u, R* r, co
   }
   const double bs = block_sum(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  if (last_block(&st->ticket_upd, nblocks, &last)) {
-    const double rr = sum_partials(part, nblocks, red);
-    if (threadIdx.x == 0) {
-      st->old = st->rr;
-      st->rr = rr;
-      st->it += 1;
-      st->ticket_upd = 0;
-      const bool go = rr > st->tol2 && (st->rel2 <= 0.0 || rr > st->rel2 * st->rr0) && st->it < st->max_it;
-      st->done = go ? 0 : 1;
-    }
-  }
 }
 
-// r.r of the initial residual and state initialisation.
+// r.r partials of the initial residual.
 template <typename R>
-__global__ __launch_bounds__(kBlk) void k_cg_init(const R* r, long long n, CgState* st, double* part, unsigned int nblocks,
-                                                  double tol2, double rel2, int max_it) {
+__global__ __launch_bounds__(kBlk) void k_cg_init(const R* __restrict__ r, long long n, double* part) {
   __shared__ double red[16];
-  __shared__ bool last;
   double acc = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     acc += (double)r[i] * (double)r[i];
   const double bs = block_sum(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = bs;
-  if (last_block(&st->ticket_upd, nblocks, &last)) {
-    const double rr = sum_partials(part, nblocks, red);
-    if (threadIdx.x == 0) {
-      st->rr = rr;
-      st->rr0 = rr;
-      st->old = 1.0;  // clean_div_CG.m:43
-      st->alpha = 0.0;
-      st->it = 0;
-      st->max_it = max_it;
-      st->tol2 = tol2;
-      st->rel2 = rel2;
-      st->ticket_upd = 0;
-      st->ticket_lap = 0;
-      st->done = (rr > tol2 && (rel2 <= 0.0 || rr > rel2 * rr) && max_it > 0) ? 0 : 1;
-    }
+}
+
+// State initialisation from the partials of k_cg_init.
+__global__ __launch_bounds__(256) void k_cg_init_state(const double* part, unsigned int n, CgState* st, double tol2,
+                                                       double rel2, int max_it) {
+  __shared__ double red[16];
+  const double rr = sum_partials(part, n, red);
+  if (threadIdx.x == 0) {
+    st->rr = rr;
+    st->rr0 = rr;
+    st->old = 1.0;  // clean_div_CG.m:43
+    st->alpha = 0.0;
+    st->it = 0;
+    st->max_it = max_it;
+    st->tol2 = tol2;
+    st->rel2 = rel2;
+    st->done = (rr > tol2 && (rel2 <= 0.0 || rr > rel2 * rr) && max_it > 0) ? 0 : 1;
   }
 }
 
@@ -296,7 +287,8 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
   // r = div B on the interior (calc_div_divcleaning, kernel_div_cleaning.cl:60-70)
   const uint32_t g_dc[3] = {(uint32_t)(p.nx - 2 * p.bnodes), (uint32_t)(p.ny - 2 * p.bnodes), (uint32_t)(p.nz - 2 * p.bnodes)};
   gen_div<R>(p, info.opset, s, g_dc, b, r, p.bnodes);
-  k_cg_init<R><<<vec_blocks, kBlk, 0, s>>>(r, n, st, part, vec_blocks, abs_thr * abs_thr, rel_tol * rel_tol, max_it);
+  k_cg_init<R><<<vec_blocks, kBlk, 0, s>>>(r, n, part);
+  k_cg_init_state<<<1, 256, 0, s>>>(part, vec_blocks, st, abs_thr * abs_thr, rel_tol * rel_tol, max_it);
 
   LapArgs la;
   la.nx = p.nx;
@@ -329,7 +321,9 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
         case 6: launch_lap<R, 6, 8>(la, u, r, un, q, st, part, s); break;
         default: launch_lap<R, 7, 8>(la, u, r, un, q, st, part, s); break;
       }
-      k_cg_update<R><<<vec_blocks, kBlk, 0, s>>>(phi, un, r, q, n, st, part, vec_blocks);
+      k_cg_alpha<<<1, 256, 0, s>>>(part, lap_blocks, st);
+      k_cg_update<R><<<vec_blocks, kBlk, 0, s>>>(phi, un, r, q, n, st, part);
+      k_cg_rr<<<1, 256, 0, s>>>(part, vec_blocks, st);
     }
     launched += k;
     if (hipMemcpyAsync(&hst, st, sizeof(CgState), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)

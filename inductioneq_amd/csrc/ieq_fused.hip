@@ -472,8 +472,10 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
   }
 }
 
+// FP64: 2 waves per SIMD (the stage state needs ~200 VGPRs); FP32 fits 4 waves without spills
 template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ, int NBUF>
-__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(IEQ_STAGE_WAVES, 8))) void k_stage(FArgs<R> a) {
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : IEQ_STAGE_WAVES, 8))) void k_stage(
+    FArgs<R> a) {
   __shared__ StageLds<R, RR, TY, HALL, NBUF> L;
   int x0, y0, z0;
   tile_origin<TY>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
@@ -481,7 +483,7 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(IEQ_STA
 }
 
 template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV>
-__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(IEQ_STAGE_WAVES, 8))) void k_stage_batched(
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : IEQ_STAGE_WAVES, 8))) void k_stage_batched(
     const FArgs<R>* __restrict__ args, const int4* __restrict__ tiles, int ntiles) {
   __shared__ StageLds<R, RR, TY, HALL> L;
   const int chunk = (ntiles + kXcds - 1) / kXcds, slots = gridDim.x / kXcds;  // gridDim.x is a multiple of 8
