@@ -18,6 +18,7 @@
 #include "ieq_cg.hpp"
 #include "ieq_device.hpp"
 #include "ieq_generic.hpp"
+#include "ieq_stencil.hpp"
 
 namespace ieq {
 
@@ -31,6 +32,7 @@ struct CgState {
 namespace {
 
 constexpr int kBlk = 256;
+constexpr int kLapRows = 32;  // table rows held in LDS by k_cg_lap (2 * NUM_BOUNDS + 1 of the Laplacian)
 
 // Deterministic block reduction of one double (wave64 shuffles, then LDS).
 __device__ inline double block_sum(double v, double* smem) {
@@ -84,12 +86,11 @@ __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restri
   const int x = a.off + blockIdx.x * 64 + tx, y = a.off + blockIdx.y * TY + ty;
   const int x0 = a.off + blockIdx.x * 64, y0 = a.off + blockIdx.y * TY;
   const bool active = (x - a.off) < gx && (y - a.off) < gy;
-  const long long NX = a.nx, NXY = (long long)a.nx * a.ny;
   // u' at a node (0 outside the grid)
   auto dirv = [&](int xx, int yy, int zz) -> R {
-    if (xx >= 0 && xx < a.nx && yy >= 0 && yy < a.ny && zz >= 0 && zz < a.nz) {
-      const long long id = xx + NX * yy + NXY * zz;
-      return beta * u[id] + r[id];
+    if ((unsigned)xx < (unsigned)a.nx && (unsigned)yy < (unsigned)a.ny && (unsigned)zz < (unsigned)a.nz) {
+      const unsigned o = (unsigned)(xx + a.nx * (yy + a.ny * zz)) * (unsigned)sizeof(R);
+      return beta * ldo(u, o) + ldo(r, o);
     }
     return R(0);
   };
@@ -103,6 +104,15 @@ __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restri
       const int j = c0 - HW + k;
       cx[k] = (j >= 0 && j < a.wl) ? (R)(a.tab[rx * a.wl + j] * a.idx2) : R(0);
       cy[k] = (j >= 0 && j < a.wl) ? (R)(a.tab[ry * a.wl + j] * a.idy2) : R(0);
+    }
+  }
+  // z rows of the table, scaled by 1/dz^2 and rounded to REAL once (uniform per plane)
+  __shared__ R sz[kLapRows][Q];
+  {
+    const int rows = 2 * a.nbl + 1, c0 = (a.wl - 1) / 2;
+    for (int i = ty * 64 + tx; i < rows * Q; i += 64 * TY) {
+      const int row = i / Q, k = i % Q, j = c0 - HW + k;
+      sz[row][k] = (j >= 0 && j < a.wl) ? (R)(a.tab[row * a.wl + j] * a.idz2) : R(0);
     }
   }
   double dot = 0.0;
@@ -133,17 +143,12 @@ __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restri
     __syncthreads();
     if (active) {
       const int rz = bound_row((uint32_t)z, (uint32_t)a.nz, a.nbl, 0);
-      const int c0 = (a.wl - 1) / 2;
       R acc = R(0);
 #pragma unroll
-      for (int k = 0; k < Q; ++k) {
-        const int j = c0 - HW + k;
-        const R cz = (j >= 0 && j < a.wl) ? (R)(a.tab[rz * a.wl + j] * a.idz2) : R(0);
-        acc = acc + cx[k] * sp[ty + HW][tx + k] + cy[k] * sp[ty + k][tx + HW] + cz * qz[k];
-      }
-      const long long id = x + NX * y + NXY * z;
-      un[id] = qz[HW];
-      q[id] = acc;
+      for (int k = 0; k < Q; ++k) acc = acc + cx[k] * sp[ty + HW][tx + k] + cy[k] * sp[ty + k][tx + HW] + sz[rz][k] * qz[k];
+      const unsigned o = (unsigned)(x + a.nx * (y + a.ny * z)) * (unsigned)sizeof(R);
+      sto(un, o, qz[HW]);
+      sto(q, o, acc);
       dot += (double)qz[HW] * (double)acc;
     }
 #pragma unroll
@@ -249,7 +254,8 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
              R* b, R* phi, double abs_thr, double rel_tol, int max_it, int chunk, ieq_cg_result* res, std::string& err) {
   const long long n = info.num_nodes;
   const int hw = (wl - 1) / 2;
-  if (hw < 1 || hw > 7 || info.periodic) return 1;  // not covered by the fused loop
+  if (hw < 1 || hw > 7 || info.periodic || 2 * nbl + 1 > kLapRows || !offsets_fit(p.nx, p.ny, p.nz, sizeof(R)))
+    return 1;  // not covered by the fused loop
   const int TY = 8;
   const int gx = p.nx - 2 * p.bnodes, gy = p.ny - 2 * p.bnodes, gz = p.nz - 2 * p.bnodes;
   const int zc = gz >= 128 ? 16 : 8;
