@@ -188,3 +188,24 @@ def test_fp32_gradient_close_to_fp64():
     assert float((g32 - g64).abs().max() / g64.abs().max()) <= 1e-4
     assert abs(gc32 - gc64) <= 1e-3 * abs(gc64)
     assert np.abs(gu32 - gu64).max() <= 1e-3 * np.abs(gu64).max()
+
+
+def test_autograd_function_matches_solver():
+    """hall_solve (torch.autograd.Function) returns the solver's gradients for b0, u and c_H."""
+    import torch
+
+    from inductioneq_amd.adjoint import hall_solve
+
+    cfg, prog, solver, b0, tgt = _setup(nsteps=4, snapshots=1)
+    u0, ch0 = [0.3, -0.2, 0.1], 0.8
+    h3 = cfg.dx * cfg.dy * cfg.dz
+    b = b0.clone().requires_grad_(True)
+    u = torch.tensor(u0, dtype=torch.float64, requires_grad=True)
+    c = torch.tensor(ch0, dtype=torch.float64, requires_grad=True)
+    bT = hall_solve(b, u, c, solver)
+    loss = 0.5 * h3 * ((bT - tgt) ** 2).sum()
+    loss.backward()
+    g, gc, gu = _grad(cfg, solver, b0, tgt, u0, ch0)
+    assert torch.equal(b.grad, g)
+    assert float(c.grad) == gc and [float(v) for v in u.grad] == gu
+    assert abs(float(loss) - _loss(cfg, solver.forward(b0, u0, ch0), tgt)) <= 1e-12 * float(loss)
