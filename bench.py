@@ -523,7 +523,7 @@ def bench_cg(a):
     r64, r32 = res["double"], res["float"]
     it_s = r32["seconds"] / max(r32["iterations"], 1)
     cells = (n - 2) ** 3
-    alg = 40 * cells  # FP32 fields: K1 reads u, r, writes u, q; K2 reads phi, u, r, q, writes phi, r (10 scalars)
+    alg = 40 * cells  # FP32 fields: k_cg_lap reads u, r, writes u', q; k_cg_update reads phi, u', r, q, writes phi, r
     diff = float((r32["b"] - r64["b"]).abs().max() / r64["b"].abs().max())
     out = {
         "metric": "CG iterations/s",
@@ -546,7 +546,7 @@ def bench_cg(a):
         "fp32_vs_fp64_cleaned_field_max_rel_diff": diff,
         "roofline": {"bound": "hbm", "achieved": round(alg / it_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(alg / it_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "one CG iteration (k_cg_dir + k_cg_lap + k_cg_update), 40 B/cell algorithmic FP32"},
+                     "kernel": "one CG iteration (k_cg_lap + k_cg_update + two one-workgroup reductions), 40 B/cell algorithmic FP32"},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -12,8 +12,10 @@
 // result is deterministic.  Once the stop test holds, every kernel
 // returns immediately, so any number of iterations can be enqueued without a host
 // round trip and the result equals stopping exactly at convergence.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "ieq_cg.hpp"
 #include "ieq_device.hpp"
@@ -71,9 +73,15 @@ struct LapArgs {
 // u' is written to a second buffer (neighbouring blocks still read u), so the direction
 // registers alternate between iterations.  Outside the interior u and r are 0 (never
 // written), so u' is 0 there as in the reference, where calc_axpy runs on the interior
-// only; neighbours outside the grid read 0 (their coefficients are 0).  Planes are
-// double-buffered in LDS and the loads of plane z+1 are in flight while plane z is computed.
-template <typename R, int HW, int TY>
+// only; neighbours outside the grid count as 0 (their coefficients are 0).
+// z-march: u' of the own column's z-stencil lives in a register window, one xy-plane plus
+// halo at a time in LDS (double-buffered).  Loads are issued PD planes ahead of their use:
+// they always target a valid node (coordinates clamped into the grid) and land in raw
+// (u, r) registers; the masking and beta u + r happen only when a plane is used, so no
+// load is waited for before it is needed.  The per-plane barrier orders LDS traffic only.
+// Tiles are handed out per XCD in contiguous ranges (x fastest, then y, then z-chunk), so
+// the halo rows of y-neighbours are read through one L2.
+template <typename R, int HW, int TY, int PD>
 __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restrict__ u, const R* __restrict__ r,
                                                     R* __restrict__ un, R* __restrict__ q, CgState* st, double* part) {
   constexpr int LX = 64 + 2 * HW, LY = TY + 2 * HW, Q = 2 * HW + 1;
@@ -83,27 +91,60 @@ __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restri
   const R beta = (R)(st->rr / st->old);
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int gx = a.nx - 2 * a.off, gy = a.ny - 2 * a.off, gz = a.nz - 2 * a.off;
-  const int x = a.off + blockIdx.x * 64 + tx, y = a.off + blockIdx.y * TY + ty;
-  const int x0 = a.off + blockIdx.x * 64, y0 = a.off + blockIdx.y * TY;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int ntx = (gx + 63) / 64, nty = (gy + TY - 1) / TY;
+  const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
+  const int x = a.off + bx * 64 + tx, y = a.off + by * TY + ty;
+  const int x0 = a.off + bx * 64, y0 = a.off + by * TY;
   const bool active = (x - a.off) < gx && (y - a.off) < gy;
-  // u' at a node (0 outside the grid)
-  auto dirv = [&](int xx, int yy, int zz) -> R {
-    if ((unsigned)xx < (unsigned)a.nx && (unsigned)yy < (unsigned)a.ny && (unsigned)zz < (unsigned)a.nz) {
-      const unsigned o = (unsigned)(xx + a.nx * (yy + a.ny * zz)) * (unsigned)sizeof(R);
-      return beta * ldo(u, o) + ldo(r, o);
-    }
-    return R(0);
-  };
+  const unsigned RB = sizeof(R);
+  const unsigned plb = (unsigned)(a.nx * a.ny) * RB;
+  auto cl = [](int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); };
+  auto inb = [](int v, int n) { return (unsigned)v < (unsigned)n; };
+  // In-plane byte offsets (clamped) and in-grid masks of the column and its halo cells.
+  // Every thread issues the same loads each plane: one x-halo cell (left or right; threads
+  // off the x edges load their own column again), one y-halo cell per NYH (below / above,
+  // or the own column), so the memory counter of a plane's loads is the same on every path.
+  constexpr int NYH = 2 * HW <= TY ? 1 : 2;
+  const unsigned oc = (unsigned)(cl(x, a.nx) + a.nx * cl(y, a.ny)) * RB;
+  const bool colin = inb(x, a.nx) && inb(y, a.ny);
+  const bool hl = tx < HW, hr = tx >= 64 - HW, hb = ty < HW, ht = ty >= TY - HW;
+  const int hxx = hl ? x0 - HW + tx : x + HW;  // x-halo cell (used when hl || hr)
+  const int hyb = y0 - HW + ty, hya = y + HW;    // y-halo cells below / above
+  const unsigned ohx = (hl || hr) ? (unsigned)(cl(hxx, a.nx) + a.nx * cl(y, a.ny)) * RB : oc;
+  unsigned ohy[NYH];
+  bool mhy[NYH];
+  if constexpr (NYH == 1) {
+    const int yy = hb ? hyb : hya;
+    ohy[0] = (hb || ht) ? (unsigned)(cl(x, a.nx) + a.nx * cl(yy, a.ny)) * RB : oc;
+    mhy[0] = inb(x, a.nx) && inb(yy, a.ny);
+  } else {
+    ohy[0] = hb ? (unsigned)(cl(x, a.nx) + a.nx * cl(hyb, a.ny)) * RB : oc;
+    ohy[1] = ht ? (unsigned)(cl(x, a.nx) + a.nx * cl(hya, a.ny)) * RB : oc;
+    mhy[0] = inb(x, a.nx) && inb(hyb, a.ny);
+    mhy[1] = inb(x, a.nx) && inb(hya, a.ny);
+  }
+  const bool mhx = inb(hxx, a.nx) && inb(y, a.ny);
+  // LDS destinations of the halo cells
+  const int lxc = hl ? tx : tx + 2 * HW;                   // x-halo column
+  const int lyb = ty, lya = ty + 2 * HW;                   // y-halo rows below / above
   // per-thread x/y rows
   R cx[Q], cy[Q];
   {
     const int rx = bound_row((uint32_t)x, (uint32_t)a.nx, a.nbl, 0), ry = bound_row((uint32_t)y, (uint32_t)a.ny, a.nbl, 0);
     const int c0 = (a.wl - 1) / 2;
+    double tcx[Q], tcy[Q];  // unconditional loads (clamped column), masked below
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      const int j = min(max(c0 - HW + k, 0), a.wl - 1);
+      tcx[k] = a.tab[rx * a.wl + j];
+      tcy[k] = a.tab[ry * a.wl + j];
+    }
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
       const int j = c0 - HW + k;
-      cx[k] = (j >= 0 && j < a.wl) ? (R)(a.tab[rx * a.wl + j] * a.idx2) : R(0);
-      cy[k] = (j >= 0 && j < a.wl) ? (R)(a.tab[ry * a.wl + j] * a.idy2) : R(0);
+      cx[k] = (j >= 0 && j < a.wl) ? (R)(tcx[k] * a.idx2) : R(0);
+      cy[k] = (j >= 0 && j < a.wl) ? (R)(tcy[k] * a.idy2) : R(0);
     }
   }
   // z rows of the table, scaled by 1/dz^2 and rounded to REAL once (uniform per plane)
@@ -116,47 +157,92 @@ __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restri
     }
   }
   double dot = 0.0;
-  R qz[Q + 1];
-  R h[4];  // halo values of the plane being staged: x left, x right, y below, y above
-  const int z0 = a.off + blockIdx.z * a.zc;
-  const int zn = min(a.zc, gz - (int)blockIdx.z * a.zc);
-  auto load_halo = [&](int z) {
-    if (tx < HW) h[0] = dirv(x0 - HW + tx, y, z);
-    if (tx >= 64 - HW) h[1] = dirv(x + HW, y, z);
-    if (ty < HW) h[2] = dirv(x, y0 - HW + ty, z);
-    if (ty >= TY - HW) h[3] = dirv(x, y + HW, z);
+  const int z0 = a.off + bz * a.zc;
+  const int zn = min(a.zc, gz - bz * a.zc);
+  // u' from raw values (0 outside the grid)
+  auto dir = [&](R uu, R rr, bool in) { return in ? beta * uu + rr : R(0); };
+  // Raw loads ride in NS = PD+1 register slots that rotate with the plane index: the
+  // column value of plane z0+HW+1+m and the halo of plane z0+m sit in slot m % NS.  The
+  // z-loop is unrolled NS times so every slot index is a compile-time constant: a register
+  // with a load in flight is never copied (a copy would wait for the load).
+  constexpr int NS = PD + 1;
+  static_assert(NS >= 2 && NS <= 4, "1 <= PD <= 3");
+  constexpr int NH = 1 + NYH;      // halo cells per thread and plane: x, then y
+  R w[Q];                          // u' of the column, planes z-HW .. z+HW
+  R lu[NS], lr[NS];                // raw column values
+  R hu[NS][NH], hrw[NS][NH];       // raw halo values
+  auto load_col = [&](R& uu, R& rr, int zz) {
+    const unsigned o = oc + (unsigned)cl(zz, a.nz) * plb;
+    uu = ldo(u, o);
+    rr = ldo(r, o);
+  };
+  auto load_halo = [&](R* uu, R* rr, int zz) {  // unconditional: every path issues 2*NH loads
+    const unsigned pz = (unsigned)cl(zz, a.nz) * plb;
+    uu[0] = ldo(u, ohx + pz);
+    rr[0] = ldo(r, ohx + pz);
+#pragma unroll
+    for (int k = 0; k < NYH; ++k) {
+      uu[1 + k] = ldo(u, ohy[k] + pz);
+      rr[1 + k] = ldo(r, ohy[k] + pz);
+    }
   };
 #pragma unroll
-  for (int j = 0; j < Q; ++j) qz[j] = dirv(x, y, z0 - HW + j);
-  load_halo(z0);
-  for (int zi = 0; zi < zn; ++zi) {
+  for (int j = 0; j < Q; ++j) {
+    R uu, rr;
+    load_col(uu, rr, z0 - HW + j);
+    w[j] = dir(uu, rr, colin && inb(z0 - HW + j, a.nz));
+  }
+  // first PD planes in flight, in the order the z-loop issues them (halo, then column); the
+  // scheduling barriers keep that order, which the memory-counter waits in the loop rely on
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int m = 0; m < PD; ++m) {
+    load_halo(hu[m], hrw[m], z0 + m);
+    load_col(lu[m], lr[m], z0 + HW + 1 + m);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  auto body = [&](auto slot, int zi) {
+    constexpr int S = decltype(slot)::value, SN = (S + PD) % NS;
     const int z = z0 + zi;
     auto& sp = s[zi & 1];
-    sp[ty + HW][tx + HW] = qz[HW];
-    if (tx < HW) sp[ty + HW][tx] = h[0];
-    if (tx >= 64 - HW) sp[ty + HW][tx + 2 * HW] = h[1];
-    if (ty < HW) sp[ty][tx + HW] = h[2];
-    if (ty >= TY - HW) sp[ty + 2 * HW][tx + HW] = h[3];
-    // loads of the next plane (halo first), in flight across the barrier and the compute
-    if (zi + 1 < zn) load_halo(z + 1);
-    qz[Q] = dirv(x, y, z + HW + 1);
-    __syncthreads();
-    if (active) {
+    sp[ty + HW][tx + HW] = w[HW];
+    if (hl || hr) sp[ty + HW][lxc] = dir(hu[S][0], hrw[S][0], mhx);
+    if constexpr (NYH == 1) {
+      if (hb || ht) sp[hb ? lyb : lya][tx + HW] = dir(hu[S][1], hrw[S][1], mhy[0]);
+    } else {
+      if (hb) sp[lyb][tx + HW] = dir(hu[S][1], hrw[S][1], mhy[0]);
+      if (ht) sp[lya][tx + HW] = dir(hu[S][2], hrw[S][2], mhy[1]);
+    }
+    // loads PD planes ahead (halo first), in flight across PD barriers and computes; past the
+    // chunk they read a clamped plane and are never used
+    load_halo(hu[SN], hrw[SN], z + PD);
+    load_col(lu[SN], lr[SN], z + HW + 1 + PD);
+    lds_barrier();
+    if (active && zi < zn) {
       const int rz = bound_row((uint32_t)z, (uint32_t)a.nz, a.nbl, 0);
       R acc = R(0);
 #pragma unroll
-      for (int k = 0; k < Q; ++k) acc = acc + cx[k] * sp[ty + HW][tx + k] + cy[k] * sp[ty + k][tx + HW] + sz[rz][k] * qz[k];
-      const unsigned o = (unsigned)(x + a.nx * (y + a.ny * z)) * (unsigned)sizeof(R);
-      sto(un, o, qz[HW]);
+      for (int k = 0; k < Q; ++k) acc = acc + cx[k] * sp[ty + HW][tx + k] + cy[k] * sp[ty + k][tx + HW] + sz[rz][k] * w[k];
+      const unsigned o = oc + (unsigned)z * plb;
+      sto(un, o, w[HW]);
       sto(q, o, acc);
-      dot += (double)qz[HW] * (double)acc;
+      dot += (double)w[HW] * (double)acc;
     }
+    // advance: the window takes plane z+HW+1 (slot S, issued PD iterations ago)
 #pragma unroll
-    for (int j = 0; j < Q; ++j) qz[j] = qz[j + 1];
+    for (int j = 0; j < Q - 1; ++j) w[j] = w[j + 1];
+    w[Q - 1] = dir(lu[S], lr[S], colin && inb(z + HW + 1, a.nz));
+  };
+  // Whole groups of NS planes (the host makes the chunk a multiple of NS; a shorter last
+  // chunk runs its trailing planes without output), so every path issues the same loads.
+  for (int zb = 0; zb < zn; zb += NS) {
+    body(std::integral_constant<int, 0>{}, zb);
+    body(std::integral_constant<int, 1 % NS>{}, zb + 1);
+    if constexpr (NS > 2) body(std::integral_constant<int, 2 % NS>{}, zb + 2);
+    if constexpr (NS > 3) body(std::integral_constant<int, 3 % NS>{}, zb + 3);
   }
   const double bs = block_sum(dot, red);
-  const unsigned int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  if (tx == 0 && ty == 0) part[bid] = bs;
+  if (tx == 0 && ty == 0) part[tile] = bs;
 }
 
 // One workgroup: alpha = rr / u'.q from the block partials (summed in block order).  A
@@ -229,11 +315,33 @@ __global__ __launch_bounds__(256) void k_cg_init_state(const double* part, unsig
   }
 }
 
+#ifndef IEQ_LAP_PD
+#define IEQ_LAP_PD 2
+#endif
+constexpr int kLapPD = IEQ_LAP_PD;  // planes of the Laplacian's z-march in flight per thread
+
 template <typename R, int HW, int TY>
 void launch_lap(const LapArgs& a, const R* u, const R* r, R* un, R* q, CgState* st, double* part, hipStream_t s) {
   const int gx = a.nx - 2 * a.off, gy = a.ny - 2 * a.off, gz = a.nz - 2 * a.off;
-  dim3 grid((gx + 63) / 64, (gy + TY - 1) / TY, (gz + a.zc - 1) / a.zc);
-  k_cg_lap<R, HW, TY><<<grid, dim3(64, TY), 0, s>>>(a, u, r, un, q, st, part);
+  const int grid = ((gx + 63) / 64) * ((gy + TY - 1) / TY) * ((gz + a.zc - 1) / a.zc);
+  k_cg_lap<R, HW, TY, kLapPD><<<grid, dim3(64, TY), 0, s>>>(a, u, r, un, q, st, part);
+}
+
+// Planes per z-chunk: the smallest chunk whose grid fits in one round of resident
+// workgroups (so no partial second round), at least 8 planes, rounded up to the kernel's
+// group of PD+1 planes.
+template <typename R, int HW, int TY> int lap_zc(int gx, int gy, int gz) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_lap<R, HW, TY, kLapPD>, 64 * TY, 0) != hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return 16;
+  const long long xy = (long long)((gx + 63) / 64) * ((gy + TY - 1) / TY);
+  const long long resident = (long long)cus * per_cu;
+  const long long chunks = std::max(1LL, resident / std::max(1LL, xy));
+  const int ns = kLapPD + 1;  // the kernel's z-loop advances in groups of ns planes
+  const int zc = std::max(8, (int)((gz + chunks - 1) / chunks));
+  return (zc + ns - 1) / ns * ns;
 }
 
 }  // namespace
@@ -258,7 +366,16 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
     return 1;  // not covered by the fused loop
   const int TY = 8;
   const int gx = p.nx - 2 * p.bnodes, gy = p.ny - 2 * p.bnodes, gz = p.nz - 2 * p.bnodes;
-  const int zc = gz >= 128 ? 16 : 8;
+  int zc = 8;
+  switch (hw) {
+    case 1: zc = lap_zc<R, 1, 8>(gx, gy, gz); break;
+    case 2: zc = lap_zc<R, 2, 8>(gx, gy, gz); break;
+    case 3: zc = lap_zc<R, 3, 8>(gx, gy, gz); break;
+    case 4: zc = lap_zc<R, 4, 8>(gx, gy, gz); break;
+    case 5: zc = lap_zc<R, 5, 8>(gx, gy, gz); break;
+    case 6: zc = lap_zc<R, 6, 8>(gx, gy, gz); break;
+    default: zc = lap_zc<R, 7, 8>(gx, gy, gz); break;
+  }
   const unsigned int lap_blocks = (unsigned)(((gx + 63) / 64) * ((gy + TY - 1) / TY) * ((gz + zc - 1) / zc));
   const unsigned int vec_blocks = 1024;
   const size_t fbytes = ((size_t)info.stride * sizeof(R) + 255) / 256 * 256;

@@ -1,0 +1,436 @@
+// Ensembles: many independent periodic solves stepped together (ieq_ensemble_* of the C ABI).
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "ieq_device.hpp"
+#include "ieq_fused.hpp"
+#include "ieq_fused_tiles.hpp"
+#include "ieq_generic.hpp"
+#include "ieq_stencil.hpp"
+
+namespace ieq {
+
+// ---------------------------------------------------------------------------------
+// Ensembles: many independent periodic solves stepped together.  Every fused kernel of
+// an RK stage is one persistent launch whose blocks stride over a tile list of all
+// members; each tile carries its member index and the member's stage arguments are an
+// FArgs entry in device memory.  One step (two for the 2N schemes, whose ping-pong
+// register alternates) is captured once in a hipGraph and replayed.
+
+struct Ensemble {
+  int real_bytes = 8;
+  int rr = 3, hall = 0, hod = 0, conv = 0, src_zero = 0;
+  StencilCoef<double> coef;  // central-row coefficients (converted to REAL per member)
+  std::vector<ieq_member> m;
+  std::vector<StageSpec> plan;
+  int nf = 0, steps_per_graph = 1;
+  std::vector<std::vector<void*>> b1_at;  // per unrolled stage: b1 pointers after swaps (2N)
+  void* dargs = nullptr;                  // FArgs<R>[nstage_unrolled][nmembers]
+  size_t args_bytes = 0;
+  int4* dtiles = nullptr;
+  int ntiles = 0, ty = 8, zc = 8, order = 0, grid_emf = 0, grid_stage = 0;
+  bool tuned = false;
+  bool use_graph = true;  // replay the captured step, or launch directly (chosen by timing)
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int ncu = 256;
+  bool final_in_alt = false;  // 2N with an odd number of stages per graph
+};
+
+namespace {
+
+template <typename R> FArgs<R> member_base(const Ensemble& e, const ieq_member& mb) {
+  FArgs<R> a{};
+  a.nx = mb.nx;
+  a.ny = mb.ny;
+  a.nz = mb.nz;
+  a.zc = e.zc;
+  a.S = (((long long)mb.nx * mb.ny * mb.nz) + 63) / 64 * 64;
+  for (int k = 0; k < 4; ++k) {
+    a.c.dp[k] = (R)e.coef.dp[k];
+    a.c.dm[k] = (R)e.coef.dm[k];
+    a.c.w[k] = (R)e.coef.w[k];
+  }
+  const double h[3] = {mb.dx, mb.dy, mb.dz};
+  for (int d = 0; d < 3; ++d) {
+    a.inv_h[d] = (R)(1.0 / h[d]);
+    a.two_h[d] = (R)(2.0 / h[d]);
+    a.h[d] = (R)h[d];
+  }
+  a.rho = (R)mb.rho;
+  a.u[0] = (R)mb.ux;
+  a.u[1] = (R)mb.uy;
+  a.u[2] = (R)mb.uz;
+  a.conv = e.conv;
+  a.src_zero = e.src_zero;
+  return a;
+}
+
+// FArgs of every unrolled stage entry for every member (zc taken from e.zc).
+template <typename R> std::vector<FArgs<R>> ensemble_args(Ensemble& e) {
+  const int nm = (int)e.m.size();
+  std::vector<FArgs<R>> out;
+  std::vector<R*> b1(nm), b1alt(nm);
+  for (int i = 0; i < nm; ++i) {
+    b1[i] = (R*)e.m[i].b1;
+    b1alt[i] = (R*)e.m[i].b3;
+  }
+  for (int step = 0; step < e.steps_per_graph; ++step)
+    for (const StageSpec& st : e.plan) {
+      for (int i = 0; i < nm; ++i) {
+        const ieq_member& mb = e.m[i];
+        R* F[3] = {b1[i], (R*)mb.b2, (R*)mb.b3};
+        FArgs<R> a = member_base<R>(e, mb);
+        if (st.kind == 3) {
+          a.Y = st.Y == 0 ? b1[i] : F[st.Y];
+          a.Eout = (R*)mb.emf;
+        } else if (st.kind == 0) {
+          a.Y = F[st.Y];
+          a.E = (R*)mb.emf;
+          a.mode = 0;
+          a.out = F[st.out];
+          a.P = st.P >= 0 ? F[st.P] : nullptr;
+          a.Q = st.Q >= 0 ? F[st.Q] : nullptr;
+          a.cP = (R)st.cP;
+          a.cQ = (R)st.cQ;
+          a.cD = (R)(st.cD * mb.dt);
+        } else {  // 2N
+          a.Y = b1[i];
+          a.E = (R*)mb.emf;
+          a.mode = 1;
+          a.b2 = (R*)mb.b2;
+          a.out = b1alt[i];
+          a.cP = (R)st.cP;
+          a.cQ = (R)st.cQ;
+          a.cD = (R)mb.dt;
+        }
+        out.push_back(a);
+      }
+      if (st.kind == 4)
+        for (int i = 0; i < nm; ++i) std::swap(b1[i], b1alt[i]);
+    }
+  e.final_in_alt = b1[0] != (R*)e.m[0].b1;
+  return out;
+}
+
+std::vector<int4> ensemble_tiles(const Ensemble& e, int ty, int zc, int order) {
+  std::vector<int> idx(e.m.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+  if (order == 1)
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+      return (long long)e.m[a].nx * e.m[a].ny * e.m[a].nz > (long long)e.m[b].nx * e.m[b].ny * e.m[b].nz;
+    });
+  std::vector<int4> t;
+  for (int i : idx) {
+    const ieq_member& mb = e.m[i];
+    for (int z = 0; z < mb.nz; z += zc)
+      for (int y = 0; y < mb.ny; y += ty)
+        for (int x = 0; x < mb.nx; x += 64) t.push_back(make_int4(x, y, z, i));
+  }
+  return t;
+}
+
+template <typename R, int RR, int TY, int MODE> const void* ens_stage_kernel(const Ensemble& e) {
+  const bool h = e.hall, d = e.hod;
+  if (e.conv) {
+    if (h && d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, true, true, MODE>);
+    if (h) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, false, true, MODE>);
+    if (d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, true, true, MODE>);
+    return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, false, true, MODE>);
+  }
+  if (h && d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, true, false, MODE>);
+  if (h) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, false, false, MODE>);
+  if (d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, true, false, MODE>);
+  return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, false, false, MODE>);
+}
+
+// The EMF kernel or the stage kernel of a plan entry (stage_mode on its register indices).
+template <typename R, int RR, int TY> const void* ens_kernel_t(const Ensemble& e, const StageSpec& st) {
+  if (st.kind == 3) return reinterpret_cast<const void*>(&k_emf_batched<R, RR, TY>);
+  switch (stage_mode(st.kind == 4, st.P, st.Q, st.Y)) {
+    case 1: return ens_stage_kernel<R, RR, TY, 1>(e);
+    case 2: return ens_stage_kernel<R, RR, TY, 2>(e);
+    case 3: return ens_stage_kernel<R, RR, TY, 3>(e);
+    default: return ens_stage_kernel<R, RR, TY, 0>(e);
+  }
+}
+
+// Kernel of a plan entry (kind 3: EMF, otherwise the stage kernel) for the ensemble's settings.
+template <typename R> const void* ens_kernel(const Ensemble& e, int ty, const StageSpec& st) {
+  if (ty == 4) {
+    if (e.rr == 1) return ens_kernel_t<R, 1, 4>(e, st);
+    if (e.rr == 2) return ens_kernel_t<R, 2, 4>(e, st);
+    return ens_kernel_t<R, 3, 4>(e, st);
+  }
+  if (e.rr == 1) return ens_kernel_t<R, 1, 8>(e, st);
+  if (e.rr == 2) return ens_kernel_t<R, 2, 8>(e, st);
+  return ens_kernel_t<R, 3, 8>(e, st);
+}
+
+template <typename R> hipError_t ens_launch_steps(Ensemble& e, hipStream_t s) {
+  const FArgs<R>* base = (const FArgs<R>*)e.dargs;
+  const size_t nm = e.m.size();
+  size_t k = 0;
+  for (int step = 0; step < e.steps_per_graph; ++step)
+    for (const StageSpec& st : e.plan) {
+      const FArgs<R>* a = base + k * nm;
+      ++k;
+      void* kargs[3] = {(void*)&a, (void*)&e.dtiles, (void*)&e.ntiles};
+      const int grid = st.kind == 3 ? e.grid_emf : e.grid_stage;
+      hipError_t r = hipLaunchKernel(ens_kernel<R>(e, e.ty, st), dim3(grid), dim3(64, e.ty), kargs, 0, s);
+      if (r != hipSuccess) return r;
+    }
+  return hipSuccess;
+}
+
+template <typename R> int ens_configure(Ensemble& e, int ty, int zc, int order, std::string& err) {
+  e.ty = ty;
+  e.zc = zc;
+  e.order = order;
+  std::vector<FArgs<R>> args = ensemble_args<R>(e);
+  const size_t ab = args.size() * sizeof(FArgs<R>);
+  if (e.args_bytes < ab) {
+    if (e.dargs) (void)hipFree(e.dargs);
+    e.dargs = nullptr;
+    if (hipMalloc(&e.dargs, ab) != hipSuccess) return err = "hipMalloc failed", -4;
+    e.args_bytes = ab;
+  }
+  if (hipMemcpyAsync(e.dargs, args.data(), ab, hipMemcpyHostToDevice, e.stream) != hipSuccess) return err = "copy failed", -4;
+  std::vector<int4> tiles = ensemble_tiles(e, ty, zc, order);
+  if (e.dtiles) (void)hipFree(e.dtiles);
+  e.dtiles = nullptr;
+  if (hipMalloc(&e.dtiles, tiles.size() * sizeof(int4)) != hipSuccess) return err = "hipMalloc failed", -4;
+  if (hipMemcpyAsync(e.dtiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, e.stream) != hipSuccess)
+    return err = "copy failed", -4;
+  e.ntiles = (int)tiles.size();
+  // persistent grids: as many blocks as can be resident at once (registers and LDS bound it)
+  // (the smallest occupancy over the plan's stage kernels)
+  int occ_emf = 1, occ_stage = 1 << 30;
+  const StageSpec emf_spec{3, -1, -1, -1, 0, 0, 0, 0};
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_emf, ens_kernel<R>(e, ty, emf_spec), 64 * ty, 0) != hipSuccess)
+    return err = "occupancy query failed", -4;
+  for (const StageSpec& st : e.plan) {
+    if (st.kind == 3) continue;
+    int o = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, ens_kernel<R>(e, ty, st), 64 * ty, 0) != hipSuccess)
+      return err = "occupancy query failed", -4;
+    occ_stage = std::min(occ_stage, o);
+  }
+  if (occ_stage == 1 << 30) return err = "ensemble plan has no stage kernel", -4;
+  // multiples of 8 (one equal share of resident slots per XCD), no more than the tiles need
+  const int need = (e.ntiles + 7) / 8 * 8;
+  e.grid_emf = std::min(need, (e.ncu * std::max(1, occ_emf) + 7) / 8 * 8);
+  e.grid_stage = std::min(need, (e.ncu * std::max(1, occ_stage) + 7) / 8 * 8);
+  if (hipStreamSynchronize(e.stream) != hipSuccess) return err = "sync failed", -4;
+  return 0;
+}
+
+template <typename R> int ens_capture(Ensemble& e, std::string& err) {
+  if (e.exec) (void)hipGraphExecDestroy(e.exec);
+  if (e.graph) (void)hipGraphDestroy(e.graph);
+  e.exec = nullptr;
+  e.graph = nullptr;
+  if (hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return err = "capture begin failed", -4;
+  hipError_t r = ens_launch_steps<R>(e, e.stream);
+  if (hipStreamEndCapture(e.stream, &e.graph) != hipSuccess || r != hipSuccess) return err = "capture failed", -4;
+  if (hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0) != hipSuccess) return err = "graph instantiate failed", -4;
+  return 0;
+}
+
+size_t member_bytes(const Ensemble& e, const ieq_member& mb) {
+  return (size_t)3 * ((((size_t)mb.nx * mb.ny * mb.nz) + 63) / 64 * 64) * e.real_bytes;
+}
+
+template <typename R> int ens_tune(Ensemble& e, std::string& err) {
+  // save the registers the trial steps overwrite
+  size_t total = 0;
+  for (auto& mb : e.m) total += 3 * member_bytes(e, mb);
+  char* save = nullptr;
+  if (hipMalloc(&save, total) != hipSuccess) return err = "hipMalloc failed (tuning snapshot)", -4;
+  size_t off = 0;
+  for (auto& mb : e.m) {
+    void* f[3] = {mb.b1, mb.b2, mb.b3};
+    for (int j = 0; j < 3; ++j) {
+      (void)hipMemcpyAsync(save + off, f[j], member_bytes(e, mb), hipMemcpyDeviceToDevice, e.stream);
+      off += member_bytes(e, mb);
+    }
+  }
+  float best = 1e30f;
+  int bty = 8, bzc = 8, bord = 1;
+  hipEvent_t t0, t1;
+  (void)hipEventCreate(&t0);
+  (void)hipEventCreate(&t1);
+  const int tys[2] = {4, 8}, zcs[3] = {4, 8, 16};
+  for (int ty : tys)
+    for (int zc : zcs)
+      for (int ord = 0; ord < 2; ++ord) {
+        int rc = ens_configure<R>(e, ty, zc, ord, err);
+        if (rc) return rc;
+        hipError_t r = ens_launch_steps<R>(e, e.stream);  // warm-up
+        float ms = 1e30f;
+        for (int rep = 0; rep < 2 && r == hipSuccess; ++rep) {  // best of two timings of two steps
+          (void)hipEventRecord(t0, e.stream);
+          r = ens_launch_steps<R>(e, e.stream);
+          if (r == hipSuccess) r = ens_launch_steps<R>(e, e.stream);
+          (void)hipEventRecord(t1, e.stream);
+          (void)hipEventSynchronize(t1);
+          float t = 0.f;
+          (void)hipEventElapsedTime(&t, t0, t1);
+          ms = std::min(ms, t);
+        }
+        if (r != hipSuccess) return err = std::string("tuning launch failed: ") + hipGetErrorString(r), -4;
+        if (ms < best) {
+          best = ms;
+          bty = ty;
+          bzc = zc;
+          bord = ord;
+        }
+      }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  off = 0;
+  for (auto& mb : e.m) {
+    void* f[3] = {mb.b1, mb.b2, mb.b3};
+    for (int j = 0; j < 3; ++j) {
+      (void)hipMemcpyAsync(f[j], save + off, member_bytes(e, mb), hipMemcpyDeviceToDevice, e.stream);
+      off += member_bytes(e, mb);
+    }
+  }
+  (void)hipStreamSynchronize(e.stream);
+  int rc = ens_configure<R>(e, bty, bzc, bord, err);
+  if (rc) return rc;
+  rc = ens_capture<R>(e, err);
+  if (rc) return rc;
+  {  // graph replay against direct launches of the same period (fields restored afterwards)
+    hipEvent_t u0, u1, u2;
+    (void)hipEventCreate(&u0);
+    (void)hipEventCreate(&u1);
+    (void)hipEventCreate(&u2);
+    off = 0;
+    (void)hipGraphLaunch(e.exec, e.stream);  // warm-up
+    (void)hipEventRecord(u0, e.stream);
+    for (int k = 0; k < 2; ++k) (void)hipGraphLaunch(e.exec, e.stream);
+    (void)hipEventRecord(u1, e.stream);
+    for (int k = 0; k < 2; ++k) (void)ens_launch_steps<R>(e, e.stream);
+    (void)hipEventRecord(u2, e.stream);
+    (void)hipEventSynchronize(u2);
+    float tg = 0.f, td = 0.f;
+    (void)hipEventElapsedTime(&tg, u0, u1);
+    (void)hipEventElapsedTime(&td, u1, u2);
+    e.use_graph = tg <= td;
+    (void)hipEventDestroy(u0);
+    (void)hipEventDestroy(u1);
+    (void)hipEventDestroy(u2);
+    for (auto& mb : e.m) {
+      void* f[3] = {mb.b1, mb.b2, mb.b3};
+      for (int j = 0; j < 3; ++j) {
+        (void)hipMemcpyAsync(f[j], save + off, member_bytes(e, mb), hipMemcpyDeviceToDevice, e.stream);
+        off += member_bytes(e, mb);
+      }
+    }
+    (void)hipStreamSynchronize(e.stream);
+  }
+  (void)hipFree(save);
+  e.tuned = true;
+  return hipGetLastError() == hipSuccess ? 0 : (err = "ensemble tuning failed", -4);
+}
+
+}  // namespace
+
+int ensemble_create(const Params& p, const ieq_program_info& info, const ieq_member* members, int n, const char* integrator,
+                    Ensemble** out, std::string& err) {
+  *out = nullptr;
+  if (!p.periodic) return err = "ensembles need a periodic program", IEQ_ERR_UNSUPPORTED;
+  if (p.diss != 0 && p.diss != 1) return err = "ensembles support high-order dissipation only", IEQ_ERR_UNSUPPORTED;
+  std::string I(integrator);
+  if (I != "SSPRK33" && I != "CarpenterKennedy2N54" && I != "ToulorgeDesmet2N84F")
+    return err = "ensembles support SSPRK33, CarpenterKennedy2N54 and ToulorgeDesmet2N84F", IEQ_ERR_UNSUPPORTED;
+  Ensemble* e = new Ensemble();
+  e->real_bytes = info.real_bytes;
+  if (!fused_plan(I, p.hall, e->plan, e->nf)) {
+    delete e;
+    return err = "unknown integrator", IEQ_ERR_ARGS;
+  }
+  e->rr = central_coefs<double>(info.opset, p.ho_factor, e->coef);
+  if (e->rr < 1 || e->rr > 3) {
+    delete e;
+    return err = "ensembles need an operator whose central rows are at most 7 wide", IEQ_ERR_UNSUPPORTED;
+  }
+  e->hall = p.hall;
+  e->hod = p.diss == 1;
+  e->src_zero = p.form_source == 0;
+  for (int i = 0; i < n; ++i) {
+    const ieq_member& mb = members[i];
+    if (mb.nx < 1 || mb.ny < 1 || mb.nz < 1 || !mb.b1 || !mb.b2 || !mb.b3 || !mb.emf || !(mb.rho != 0.0) || !(mb.dx > 0) ||
+        !(mb.dy > 0) || !(mb.dz > 0)) {
+      delete e;
+      return err = "invalid member " + std::to_string(i), IEQ_ERR_ARGS;
+    }
+    if (!offsets_fit(mb.nx, mb.ny, mb.nz, info.real_bytes)) {
+      delete e;
+      return err = "member " + std::to_string(i) + " is too large (one component must stay below 2 GiB)", IEQ_ERR_UNSUPPORTED;
+    }
+    if (mb.ux != 0.0 || mb.uy != 0.0 || mb.uz != 0.0) e->conv = 1;
+    e->m.push_back(mb);
+  }
+  e->steps_per_graph = (e->plan.size() > 0 && e->plan.back().kind == 4 && (e->plan.size() / (p.hall ? 2 : 1)) % 2) ? 2 : 1;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) e->ncu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return err = "stream creation failed", IEQ_ERR_HIP;
+  }
+  *out = e;
+  return IEQ_OK;
+}
+
+int ensemble_step(Ensemble* e, long long nsteps, double* seconds, std::string& err) {
+  int rc = 0;
+  if (!e->tuned) {
+    rc = e->real_bytes == 8 ? ens_tune<double>(*e, err) : ens_tune<float>(*e, err);
+    if (rc) return IEQ_ERR_HIP;
+  }
+  if (nsteps % e->steps_per_graph) return err = "this integrator advances the ensemble in pairs of steps", IEQ_ERR_ARGS;
+  hipEvent_t t0, t1;
+  (void)hipEventCreate(&t0);
+  (void)hipEventCreate(&t1);
+  (void)hipEventRecord(t0, e->stream);
+  for (long long k = 0; k < nsteps; k += e->steps_per_graph) {
+    if (e->use_graph) {
+      if (hipGraphLaunch(e->exec, e->stream) != hipSuccess) return err = "graph launch failed", IEQ_ERR_HIP;
+    } else {
+      hipError_t r = e->real_bytes == 8 ? ens_launch_steps<double>(*e, e->stream) : ens_launch_steps<float>(*e, e->stream);
+      if (r != hipSuccess) return err = std::string("ensemble launch failed: ") + hipGetErrorString(r), IEQ_ERR_HIP;
+    }
+  }
+  (void)hipEventRecord(t1, e->stream);
+  if (hipEventSynchronize(t1) != hipSuccess) return err = "ensemble step failed", IEQ_ERR_HIP;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, t0, t1);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (seconds) *seconds = ms * 1e-3;
+  return hipGetLastError() == hipSuccess ? IEQ_OK : (err = "ensemble kernel failed", IEQ_ERR_HIP);
+}
+
+void ensemble_config(const Ensemble* e, int* ty, int* zc, int* order) {
+  *ty = e->ty;
+  *zc = e->zc;
+  *order = e->order + (e->use_graph ? 2 : 0);
+}
+
+void ensemble_release(Ensemble* e) {
+  if (!e) return;
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  if (e->dargs) (void)hipFree(e->dargs);
+  if (e->dtiles) (void)hipFree(e->dtiles);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+}  // namespace ieq

@@ -63,9 +63,14 @@ template <int TY> __device__ inline void tile_origin(int t, int nx, int ny, int 
   z0 = (t / nty) * zc;
 }
 
+// Periodic index into [0, n).  The z-marching kernels call it with indices at most a few
+// stencil widths outside the grid, so one correction usually suffices; the loops keep it
+// exact for any v without an integer division (which would cost tens of scalar
+// instructions per call).
 __device__ inline int wrap(int v, int n) {
-  v %= n;
-  return v < 0 ? v + n : v;
+  while (v < 0) v += n;
+  while (v >= n) v -= n;
+  return v;
 }
 
 template <typename R, int RR> __device__ inline R deriv(const StencilCoef<R>& c, const R* v) {
@@ -188,6 +193,22 @@ struct StageSpec {
   int out, P, Q, Y;
   double cP, cQ, cD;  // K_STAGE: cD multiplies DT; 2R: (a, b); 2N: A (cP), B (cQ)
 };
+
+// Stage-kernel variant of a stage with operands P and Q (absent: null / -1) and differentiated
+// field Y: 1 for the 2N update; 2 when only P must be loaded (Q is Y or absent); 3 when no
+// operand must be loaded (P is Y or absent, no Q); 0 otherwise.
+inline int stage_mode(bool two_n, const void* P, const void* Q, const void* Y) {
+  if (two_n) return 1;
+  if (P && P != Y && (!Q || Q == Y)) return 2;
+  if ((!P || P == Y) && !Q) return 3;
+  return 0;
+}
+inline int stage_mode(bool two_n, int P, int Q, int Y) {
+  if (two_n) return 1;
+  if (P >= 0 && P != Y && (Q < 0 || Q == Y)) return 2;
+  if ((P < 0 || P == Y) && Q < 0) return 3;
+  return 0;
+}
 
 // Fused plan of one step (same arithmetic as the reference schedule, time kernels dropped).
 bool fused_plan(const std::string& I, int hall, std::vector<StageSpec>& v, int& nf);
