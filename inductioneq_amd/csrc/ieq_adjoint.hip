@@ -26,7 +26,6 @@ namespace ieq {
 
 namespace {
 
-constexpr int kAdjTY = 8;
 
 template <typename R> struct PreArgs {
   int nx, ny, nz, zc;
@@ -257,6 +256,7 @@ struct Adjoint {
   double* partial = nullptr;
   double* acc = nullptr;
   int nblk = 0;
+  int ty = 8, zc = 8;  // tile rows and z-chunk of every launch (chosen at creation)
   Binomial bin;
   double c_hall = 1.0, u[3] = {0, 0, 0};
   bool have_forward = false;
@@ -277,7 +277,7 @@ template <typename R> FArgs<R> base_args(const Adjoint& A, const StencilCoef<R>&
   a.nx = A.p.nx;
   a.ny = A.p.ny;
   a.nz = A.p.nz;
-  a.zc = 8;
+  a.zc = A.zc;
   a.S = A.info.stride;
   a.c = coef;
   const double h[3] = {A.p.dx, A.p.dy, A.p.dz};
@@ -313,7 +313,7 @@ template <typename R> void fwd_step_to(Adjoint& A, R* b1, R* b2, R* b3, R* out) 
     if (hall) {
       a.Y = F[Ys[k]];
       a.Eout = (R*)A.E;
-      launch_emf<R>(A.rr, kAdjTY, a, A.s);
+      launch_emf<R>(A.rr, A.ty, a, A.s);
     }
     a.Y = F[Ys[k]];
     a.E = (R*)A.E;
@@ -324,7 +324,7 @@ template <typename R> void fwd_step_to(Adjoint& A, R* b1, R* b2, R* b3, R* out) 
     a.cP = (R)cPs[k];
     a.cQ = (R)cQs[k];
     a.cD = (R)(cDs[k] * A.p.dt);
-    launch_stage<R>(A.rr, kAdjTY, a, hall, hod, A.s);
+    launch_stage<R>(A.rr, A.ty, a, hall, hod, A.s);
   }
   ++A.fwd_steps;
 }
@@ -336,7 +336,7 @@ template <typename R> void pre(Adjoint& A, const R* Y, const R* L, double scale)
   q.nx = A.p.nx;
   q.ny = A.p.ny;
   q.nz = A.p.nz;
-  q.zc = 8;
+  q.zc = A.zc;
   q.S = A.info.stride;
   q.c = coefs<R>(A);
   q.h[0] = (R)A.p.dx;
@@ -351,10 +351,16 @@ template <typename R> void pre(Adjoint& A, const R* Y, const R* L, double scale)
   q.G = (R*)A.G;
   q.partial = A.partial;
   const int grid = A.nblk;
-  dim3 blk(64, kAdjTY);
-  if (A.rr == 1) k_adj_pre<R, 1, kAdjTY><<<grid, blk, 0, A.s>>>(q);
-  else if (A.rr == 2) k_adj_pre<R, 2, kAdjTY><<<grid, blk, 0, A.s>>>(q);
-  else k_adj_pre<R, 3, kAdjTY><<<grid, blk, 0, A.s>>>(q);
+  dim3 blk(64, A.ty);
+  if (A.ty == 4) {
+    if (A.rr == 1) k_adj_pre<R, 1, 4><<<grid, blk, 0, A.s>>>(q);
+    else if (A.rr == 2) k_adj_pre<R, 2, 4><<<grid, blk, 0, A.s>>>(q);
+    else k_adj_pre<R, 3, 4><<<grid, blk, 0, A.s>>>(q);
+  } else {
+    if (A.rr == 1) k_adj_pre<R, 1, 8><<<grid, blk, 0, A.s>>>(q);
+    else if (A.rr == 2) k_adj_pre<R, 2, 8><<<grid, blk, 0, A.s>>>(q);
+    else k_adj_pre<R, 3, 8><<<grid, blk, 0, A.s>>>(q);
+  }
   k_adj_accumulate<<<1, 256, 0, A.s>>>(A.partial, grid, scale, A.acc);
 }
 
@@ -381,7 +387,7 @@ template <typename R> void adj_stage(Adjoint& A, const R* L, R* out, const R* P,
     a.E = nullptr;
     a.G = nullptr;
   }
-  launch_stage<R>(A.rr, kAdjTY, a, A.p.hall != 0, A.p.diss == 1, A.s);
+  launch_stage<R>(A.rr, A.ty, a, A.p.hall != 0, A.p.diss == 1, A.s);
 }
 
 // Reverse one SSPRK33 step given its stage inputs b1, b2, b3: lam <- step^T lam, parameter partials.
@@ -412,18 +418,18 @@ template <typename R> void adj_step(Adjoint& A, const R* b1) {
   if (hall) {
     a.Y = b1;
     a.Eout = (R*)A.E;
-    launch_emf<R>(A.rr, kAdjTY, a, A.s);
+    launch_emf<R>(A.rr, A.ty, a, A.s);
   }
   a.Y = b1; a.E = (R*)A.E; a.mode = 0; a.out = b2; a.P = b1; a.Q = nullptr; a.cP = R(1); a.cQ = R(0);
   a.cD = (R)(1.0 * A.p.dt);
-  launch_stage<R>(A.rr, kAdjTY, a, hall, hod, A.s);
+  launch_stage<R>(A.rr, A.ty, a, hall, hod, A.s);
   if (hall) {
     a.Y = b2;
     a.Eout = (R*)A.E;
-    launch_emf<R>(A.rr, kAdjTY, a, A.s);
+    launch_emf<R>(A.rr, A.ty, a, A.s);
   }
   a.Y = b2; a.E = (R*)A.E; a.out = b3; a.P = b1; a.Q = b2; a.cP = (R)0.75; a.cQ = (R)0.25; a.cD = (R)(0.25 * A.p.dt);
-  launch_stage<R>(A.rr, kAdjTY, a, hall, hod, A.s);
+  launch_stage<R>(A.rr, A.ty, a, hall, hod, A.s);
   adj_from<R>(A, b1, b2, b3);
 }
 
@@ -578,7 +584,32 @@ int adjoint_create(const Params& p, const ieq_program_info& info, long long nste
       adjoint_release(A);
       return err = "hipMalloc failed (snapshots)", IEQ_ERR_HIP;
     }
-  A->nblk = ((p.nx + 63) / 64) * ((p.ny + kAdjTY - 1) / kAdjTY) * ((p.nz + 7) / 8);
+  // tiles as in ieq_step: FP64 4 rows, FP32 8; the z-chunk that fills one round of resident
+  // workgroups of the forward stage kernel (SSPRK33 stage 2 form, convection on)
+  A->ty = info.real_bytes == 8 ? 4 : 8;
+  {
+    int fake[3];  // distinct addresses standing for b1, b2, E: only compared, never read
+    if (info.real_bytes == 8) {
+      StencilCoef<double> cc;
+      central_coefs<double>(info.opset, p.ho_factor, cc);
+      FArgs<double> t = base_args<double>(*A, cc);
+      t.conv = 1;
+      t.Y = t.Q = (const double*)&fake[1];
+      t.P = (const double*)&fake[0];
+      t.E = (const double*)&fake[2];
+      A->zc = pick_zc<double>(A->rr, A->ty, t, p.hall != 0, p.diss == 1);
+    } else {
+      StencilCoef<float> cc;
+      central_coefs<float>(info.opset, p.ho_factor, cc);
+      FArgs<float> t = base_args<float>(*A, cc);
+      t.conv = 1;
+      t.Y = t.Q = (const float*)&fake[1];
+      t.P = (const float*)&fake[0];
+      t.E = (const float*)&fake[2];
+      A->zc = pick_zc<float>(A->rr, A->ty, t, p.hall != 0, p.diss == 1);
+    }
+  }
+  A->nblk = ((p.nx + 63) / 64) * ((p.ny + A->ty - 1) / A->ty) * ((p.nz + A->zc - 1) / A->zc);
   if (hipMalloc(&A->partial, (size_t)A->nblk * 4 * sizeof(double)) != hipSuccess ||
       hipMalloc(&A->acc, 4 * sizeof(double)) != hipSuccess) {
     adjoint_release(A);

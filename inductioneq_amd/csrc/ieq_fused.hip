@@ -151,27 +151,21 @@ template <typename R> __global__ void k_advance_time(R* time, R dt, long long ns
   }
 }
 
-template <typename R, int RR, int TY, int NBUF>
-void launch_emf_ty(const FArgs<R>& a, hipStream_t s) {
-  const int grid = ((a.nx + 63) / 64) * ((a.ny + TY - 1) / TY) * ((a.nz + a.zc - 1) / a.zc);
-  k_emf<R, RR, TY, NBUF><<<grid, dim3(64, TY), 0, s>>>(a);
+template <typename R, int RR, int TY, int NBUF> const void* emf_kernel_ty() {
+  return reinterpret_cast<const void*>(&k_emf<R, RR, TY, NBUF>);
 }
 
-template <typename R, int RR, int TY, bool ADJ, int NBUF, int MODE>
-void launch_stage_hdc(const FArgs<R>& a, bool hall, bool hod, bool conv, hipStream_t s) {
-  const int grid = ((a.nx + 63) / 64) * ((a.ny + TY - 1) / TY) * ((a.nz + a.zc - 1) / a.zc);
-  dim3 blk(64, TY);
+template <typename R, int RR, int TY, bool ADJ, int NBUF, int MODE> const void* stage_kernel_hdc(bool hall, bool hod, bool conv) {
   if (conv) {
-    if (hall && hod) k_stage<R, RR, TY, true, true, true, ADJ, NBUF, MODE><<<grid, blk, 0, s>>>(a);
-    else if (hall) k_stage<R, RR, TY, true, false, true, ADJ, NBUF, MODE><<<grid, blk, 0, s>>>(a);
-    else if (hod) k_stage<R, RR, TY, false, true, true, ADJ, NBUF, MODE><<<grid, blk, 0, s>>>(a);
-    else k_stage<R, RR, TY, false, false, true, ADJ, NBUF, MODE><<<grid, blk, 0, s>>>(a);
-  } else {
-    if (hall && hod) k_stage<R, RR, TY, true, true, false, ADJ, NBUF, MODE><<<grid, blk, 0, s>>>(a);
-    else if (hall) k_stage<R, RR, TY, true, false, false, ADJ, NBUF, MODE><<<grid, blk, 0, s>>>(a);
-    else if (hod) k_stage<R, RR, TY, false, true, false, ADJ, NBUF, MODE><<<grid, blk, 0, s>>>(a);
-    else k_stage<R, RR, TY, false, false, false, ADJ, NBUF, MODE><<<grid, blk, 0, s>>>(a);
+    if (hall && hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, true, true, ADJ, NBUF, MODE>);
+    if (hall) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, false, true, ADJ, NBUF, MODE>);
+    if (hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, true, true, ADJ, NBUF, MODE>);
+    return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, false, true, ADJ, NBUF, MODE>);
   }
+  if (hall && hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, true, false, ADJ, NBUF, MODE>);
+  if (hall) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, false, false, ADJ, NBUF, MODE>);
+  if (hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, true, false, ADJ, NBUF, MODE>);
+  return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, false, false, ADJ, NBUF, MODE>);
 }
 
 // LDS plane buffers of the single-grid stage kernel: 2 (one barrier per plane) or 1 (half the
@@ -179,40 +173,40 @@ void launch_stage_hdc(const FArgs<R>& a, bool hall, bool hod, bool conv, hipStre
 #ifndef IEQ_STAGE_NBUF
 #define IEQ_STAGE_NBUF 2
 #endif
-// The transposed stages of the adjoint sweeps always use the three-register form (mode 0).
-template <typename R, int RR, int TY>
-void launch_stage_ty(const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
-  if (a.G || a.src_adj || a.cY != R(0)) {  // transposed stage (adjoint sweeps): convection always evaluated
-    launch_stage_hdc<R, RR, TY, true, IEQ_STAGE_NBUF, 0>(a, hall, hod, true, s);
-    return;
-  }
+// Stage kernel for the arguments: the transposed stages of the adjoint sweeps (G, src_adj or
+// cY set) use the three-register form with convection always evaluated; otherwise the form
+// follows stage_mode.
+template <typename R, int RR, int TY> const void* stage_kernel_ty(const FArgs<R>& a, bool hall, bool hod) {
+  if (a.G || a.src_adj || a.cY != R(0)) return stage_kernel_hdc<R, RR, TY, true, IEQ_STAGE_NBUF, 0>(hall, hod, true);
   switch (stage_mode(a.mode == 1, a.P, a.Q, a.Y)) {
-    case 1: launch_stage_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 1>(a, hall, hod, a.conv, s); break;
-    case 2: launch_stage_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 2>(a, hall, hod, a.conv, s); break;
-    case 3: launch_stage_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 3>(a, hall, hod, a.conv, s); break;
-    default: launch_stage_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 0>(a, hall, hod, a.conv, s); break;
+    case 1: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 1>(hall, hod, a.conv);
+    case 2: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 2>(hall, hod, a.conv);
+    case 3: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 3>(hall, hod, a.conv);
+    default: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 0>(hall, hod, a.conv);
   }
 }
 
-template <typename R, int TY> void emf_t(int rr, const FArgs<R>& a, hipStream_t s) {
-  if (rr == 1) launch_emf_ty<R, 1, TY, 2>(a, s);
-  else if (rr == 2) launch_emf_ty<R, 2, TY, 2>(a, s);
-  else launch_emf_ty<R, 3, TY, 2>(a, s);
+template <typename R> const void* emf_kernel(int rr, int ty) {
+  if (ty == 8) return rr == 1 ? emf_kernel_ty<R, 1, 8, 2>() : rr == 2 ? emf_kernel_ty<R, 2, 8, 2>() : emf_kernel_ty<R, 3, 8, 2>();
+  return rr == 1 ? emf_kernel_ty<R, 1, 4, 2>() : rr == 2 ? emf_kernel_ty<R, 2, 4, 2>() : emf_kernel_ty<R, 3, 4, 2>();
 }
-template <typename R> void emf(int rr, int ty, const FArgs<R>& a, hipStream_t s) {
-  if (ty == 8) emf_t<R, 8>(rr, a, s);
-  else emf_t<R, 4>(rr, a, s);
+template <typename R> const void* stage_kernel(int rr, int ty, const FArgs<R>& a, bool hall, bool hod) {
+  if (ty == 8)
+    return rr == 1 ? stage_kernel_ty<R, 1, 8>(a, hall, hod) : rr == 2 ? stage_kernel_ty<R, 2, 8>(a, hall, hod) : stage_kernel_ty<R, 3, 8>(a, hall, hod);
+  return rr == 1 ? stage_kernel_ty<R, 1, 4>(a, hall, hod) : rr == 2 ? stage_kernel_ty<R, 2, 4>(a, hall, hod) : stage_kernel_ty<R, 3, 4>(a, hall, hod);
 }
+
+template <typename R> int tile_grid(const FArgs<R>& a, int ty) {
+  return ((a.nx + 63) / 64) * ((a.ny + ty - 1) / ty) * ((a.nz + a.zc - 1) / a.zc);
+}
+template <typename R> void launch_k(const void* k, int ty, const FArgs<R>& a, hipStream_t s) {
+  FArgs<R> arg = a;
+  void* kargs[1] = {(void*)&arg};
+  (void)hipLaunchKernel(k, dim3(tile_grid(a, ty)), dim3(64, ty), kargs, 0, s);
+}
+template <typename R> void emf(int rr, int ty, const FArgs<R>& a, hipStream_t s) { launch_k(emf_kernel<R>(rr, ty), ty, a, s); }
 template <typename R> void stage(int rr, int ty, const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
-  if (ty == 8) {
-    if (rr == 1) launch_stage_ty<R, 1, 8>(a, hall, hod, s);
-    else if (rr == 2) launch_stage_ty<R, 2, 8>(a, hall, hod, s);
-    else launch_stage_ty<R, 3, 8>(a, hall, hod, s);
-  } else {
-    if (rr == 1) launch_stage_ty<R, 1, 4>(a, hall, hod, s);
-    else if (rr == 2) launch_stage_ty<R, 2, 4>(a, hall, hod, s);
-    else launch_stage_ty<R, 3, 4>(a, hall, hod, s);
-  }
+  launch_k(stage_kernel<R>(rr, ty, a, hall, hod), ty, a, s);
 }
 
 int env_int(const char* k, int dflt) {
@@ -233,10 +227,26 @@ template <typename R> void launch_emf(int rr, int ty, const FArgs<R>& a, hipStre
 template <typename R> void launch_stage(int rr, int ty, const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
   stage<R>(rr, ty, a, hall, hod, s);
 }
+
+// z-chunk for a grid: the smallest chunk (at least 4 planes) whose tile grid fits one round of
+// resident workgroups of the stage kernel the arguments select, so no partial second round
+// and the shortest possible z lead-in for the available parallelism.
+template <typename R> int pick_zc(int rr, int ty, const FArgs<R>& a, bool hall, bool hod) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, stage_kernel<R>(rr, ty, a, hall, hod), 64 * ty, 0) != hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return 8;
+  const long long xy = (long long)((a.nx + 63) / 64) * ((a.ny + ty - 1) / ty);
+  const long long chunks = std::max(1LL, (long long)cus * per_cu / std::max(1LL, xy));
+  return std::max(4, (int)((a.nz + chunks - 1) / chunks));
+}
 template void launch_emf<double>(int, int, const FArgs<double>&, hipStream_t);
 template void launch_emf<float>(int, int, const FArgs<float>&, hipStream_t);
 template void launch_stage<double>(int, int, const FArgs<double>&, bool, bool, hipStream_t);
 template void launch_stage<float>(int, int, const FArgs<float>&, bool, bool, hipStream_t);
+template int pick_zc<double>(int, int, const FArgs<double>&, bool, bool);
+template int pick_zc<float>(int, int, const FArgs<float>&, bool, bool);
 
 }  // namespace ieq
 
@@ -433,11 +443,9 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
   a.nx = p.nx;
   a.ny = p.ny;
   a.nz = p.nz;
-  // Tile shape measured at 128^3: FP64 4 rows x 16 planes (2 waves per SIMD, so smaller
-  // workgroups and a shorter z lead-in), FP32 8 rows x 8 planes.
+  // Tile rows measured at 128^3: FP64 4 (2 waves per SIMD, so smaller workgroups), FP32 8.
+  // The z-chunk is set below from the stage kernel's occupancy (16 and 8 planes at 128^3).
   const bool f64 = sizeof(R) == 8;
-  a.zc = env_int("IEQ_FUSED_ZC", f64 ? 16 : 8);
-  if (a.zc < 1) a.zc = 1;
   const int ty = env_int("IEQ_FUSED_TY", f64 ? 4 : 8) == 4 ? 4 : 8;
   a.S = info.stride;
   a.c = coef;
@@ -458,6 +466,28 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
 
   R* b1 = F[0];
   R* b1alt = nf == 2 ? (R*)w->scratch : nullptr;
+  a.zc = env_int("IEQ_FUSED_ZC", 0);
+  if (a.zc < 1) {  // one round of resident workgroups for the plan's first stage kernel
+    FArgs<R> t = a;
+    t.zc = 1;
+    for (const StageSpec& st : plan) {
+      if (st.kind == 4) {
+        t.mode = 1;
+        t.Y = b1;
+        t.b2 = F[1];
+        break;
+      }
+      if (st.kind == 0) {
+        R* FF[3] = {b1, F[1], F[2]};
+        t.mode = 0;
+        t.Y = FF[st.Y];
+        t.P = st.P >= 0 ? FF[st.P] : nullptr;
+        t.Q = st.Q >= 0 ? FF[st.Q] : nullptr;
+        break;
+      }
+    }
+    a.zc = pick_zc<R>(rr, ty, t, p.hall, hod);
+  }
   for (long long step = 0; step < nsteps; ++step) {
     for (const StageSpec& st : plan) {
       a.P = nullptr;

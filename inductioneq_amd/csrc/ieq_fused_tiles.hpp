@@ -11,9 +11,11 @@
 namespace ieq {
 namespace {
 
-// Planes of column loads in flight per thread in the z-marching kernels: two when the stencil
-// half-width allows (the loads for plane p + 2 are issued while plane p is computed), else one.
-template <int RR> constexpr int march_pd() { return RR >= 2 ? 2 : 1; }
+// Planes of column loads in flight per thread in the z-marching kernels: two for FP64 when the
+// stencil half-width allows (the loads for plane p + 2 are issued while plane p is computed),
+// else one.  Measured at 128^3: FP64 (2 waves per SIMD) gains 9 % from the second plane; FP32
+// (4 waves per SIMD, batched or not) is faster with one.
+template <typename R, int RR> constexpr int march_pd() { return (sizeof(R) == 8 && RR >= 2) ? 2 : 1; }
 
 // Memory-counter discipline of the z-marching loops.  On gfx9-family GPUs one counter tracks
 // loads and stores and retires in issue order, and the compiler waits for a register's load
@@ -37,7 +39,7 @@ template <int RR> constexpr int march_pd() { return RR >= 2 ? 2 : 1; }
 // XCD) read those rows as their own column values, so the halo rows are L2 hits.
 template <typename R, int RR, int TY, int NBUF = 2>
 __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int z0, R (*sB)[3][TY + 2 * RR][64 + 2 * RR]) {
-  constexpr int Q = 2 * RR + 1, PD = march_pd<RR>();
+  constexpr int Q = 2 * RR + 1, PD = march_pd<R, RR>();
   const int tx = threadIdx.x, ty = threadIdx.y;
   const bool active = x0 + tx < a.nx && y0 + ty < a.ny;
   const unsigned RB = sizeof(R);
@@ -177,7 +179,7 @@ template <typename R, int RR, int TY, bool HALL, int NBUF = 2> struct StageLds {
 // emf_tile.
 template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ, int NBUF, int MODE>
 __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, int z0, StageLds<R, RR, TY, HALL, NBUF>& L) {
-  constexpr int Q = 2 * RR + 1, PD = march_pd<RR>();
+  constexpr int Q = 2 * RR + 1, PD = march_pd<R, RR>();
   constexpr int NC = HALL ? 6 : 3;                            // column loads: B, then E_x, E_y, E_z
   constexpr int NPW = (MODE == 0 ? 6 : MODE == 3 ? 0 : 3) + (ADJ ? 3 : 0);  // pointwise loads: P (b2), [Q], [G]
   static_assert(!ADJ || MODE == 0, "the transposed stages use mode 0");

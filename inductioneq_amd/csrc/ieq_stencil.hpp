@@ -216,5 +216,7 @@ bool fused_plan(const std::string& I, int hall, std::vector<StageSpec>& v, int& 
 // Fused forward launches (TY in {4, 8}, RR in {1, 2, 3}).
 template <typename R> void launch_emf(int rr, int ty, const FArgs<R>& a, hipStream_t s);
 template <typename R> void launch_stage(int rr, int ty, const FArgs<R>& a, bool hall, bool hod, hipStream_t s);
+// z-chunk that makes the stage kernel's tile grid one round of resident workgroups.
+template <typename R> int pick_zc(int rr, int ty, const FArgs<R>& a, bool hall, bool hod);
 
 }  // namespace ieq
