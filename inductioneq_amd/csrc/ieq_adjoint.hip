@@ -257,6 +257,7 @@ struct Adjoint {
   double* acc = nullptr;
   int nblk = 0;
   int ty = 8, zc = 8;  // tile rows and z-chunk of every launch (chosen at creation)
+  int zc_emf = 8;      // z-chunk of the EMF launches
   Binomial bin;
   double c_hall = 1.0, u[3] = {0, 0, 0};
   bool have_forward = false;
@@ -299,6 +300,11 @@ template <typename R> StencilCoef<R> coefs(const Adjoint& A) {
   return c;
 }
 
+template <typename R> void emf_launch(const Adjoint& A, FArgs<R> a) {
+  a.zc = A.zc_emf;
+  launch_emf<R>(A.rr, A.ty, a, A.s);
+}
+
 // One forward SSPRK33 step from b1 with stage registers b2, b3 (and E); the last stage writes
 // `out` (which may be b1 itself).  The reference stage arithmetic (kernel_time_integrator.cl:224-303).
 template <typename R> void fwd_step_to(Adjoint& A, R* b1, R* b2, R* b3, R* out) {
@@ -313,7 +319,7 @@ template <typename R> void fwd_step_to(Adjoint& A, R* b1, R* b2, R* b3, R* out) 
     if (hall) {
       a.Y = F[Ys[k]];
       a.Eout = (R*)A.E;
-      launch_emf<R>(A.rr, A.ty, a, A.s);
+      emf_launch<R>(A, a);
     }
     a.Y = F[Ys[k]];
     a.E = (R*)A.E;
@@ -418,7 +424,7 @@ template <typename R> void adj_step(Adjoint& A, const R* b1) {
   if (hall) {
     a.Y = b1;
     a.Eout = (R*)A.E;
-    launch_emf<R>(A.rr, A.ty, a, A.s);
+    emf_launch<R>(A, a);
   }
   a.Y = b1; a.E = (R*)A.E; a.mode = 0; a.out = b2; a.P = b1; a.Q = nullptr; a.cP = R(1); a.cQ = R(0);
   a.cD = (R)(1.0 * A.p.dt);
@@ -426,7 +432,7 @@ template <typename R> void adj_step(Adjoint& A, const R* b1) {
   if (hall) {
     a.Y = b2;
     a.Eout = (R*)A.E;
-    launch_emf<R>(A.rr, A.ty, a, A.s);
+    emf_launch<R>(A, a);
   }
   a.Y = b2; a.E = (R*)A.E; a.out = b3; a.P = b1; a.Q = b2; a.cP = (R)0.75; a.cQ = (R)0.25; a.cD = (R)(0.25 * A.p.dt);
   launch_stage<R>(A.rr, A.ty, a, hall, hod, A.s);
@@ -598,6 +604,7 @@ int adjoint_create(const Params& p, const ieq_program_info& info, long long nste
       t.P = (const double*)&fake[0];
       t.E = (const double*)&fake[2];
       A->zc = pick_zc<double>(A->rr, A->ty, t, p.hall != 0, p.diss == 1);
+      A->zc_emf = pick_zc_emf<double>(A->rr, A->ty, t);
     } else {
       StencilCoef<float> cc;
       central_coefs<float>(info.opset, p.ho_factor, cc);
@@ -607,6 +614,7 @@ int adjoint_create(const Params& p, const ieq_program_info& info, long long nste
       t.P = (const float*)&fake[0];
       t.E = (const float*)&fake[2];
       A->zc = pick_zc<float>(A->rr, A->ty, t, p.hall != 0, p.diss == 1);
+      A->zc_emf = pick_zc_emf<float>(A->rr, A->ty, t);
     }
   }
   A->nblk = ((p.nx + 63) / 64) * ((p.ny + A->ty - 1) / A->ty) * ((p.nz + A->zc - 1) / A->zc);

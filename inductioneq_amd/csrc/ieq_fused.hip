@@ -231,22 +231,28 @@ template <typename R> void launch_stage(int rr, int ty, const FArgs<R>& a, bool 
 // z-chunk for a grid: the smallest chunk (at least 4 planes) whose tile grid fits one round of
 // resident workgroups of the stage kernel the arguments select, so no partial second round
 // and the shortest possible z lead-in for the available parallelism.
-template <typename R> int pick_zc(int rr, int ty, const FArgs<R>& a, bool hall, bool hod) {
+static int zc_for(const void* kernel, int ty, int nx, int ny, int nz) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, stage_kernel<R>(rr, ty, a, hall, hod), 64 * ty, 0) != hipSuccess ||
-      cus <= 0 || per_cu <= 0)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * ty, 0) != hipSuccess || cus <= 0 || per_cu <= 0)
     return 8;
-  const long long xy = (long long)((a.nx + 63) / 64) * ((a.ny + ty - 1) / ty);
+  const long long xy = (long long)((nx + 63) / 64) * ((ny + ty - 1) / ty);
   const long long chunks = std::max(1LL, (long long)cus * per_cu / std::max(1LL, xy));
-  return std::max(4, (int)((a.nz + chunks - 1) / chunks));
+  return std::max(4, (int)((nz + chunks - 1) / chunks));
 }
+template <typename R> int pick_zc(int rr, int ty, const FArgs<R>& a, bool hall, bool hod) {
+  return zc_for(stage_kernel<R>(rr, ty, a, hall, hod), ty, a.nx, a.ny, a.nz);
+}
+// The same for the EMF kernel (fewer registers, so more resident workgroups and shorter chunks).
+template <typename R> int pick_zc_emf(int rr, int ty, const FArgs<R>& a) { return zc_for(emf_kernel<R>(rr, ty), ty, a.nx, a.ny, a.nz); }
 template void launch_emf<double>(int, int, const FArgs<double>&, hipStream_t);
 template void launch_emf<float>(int, int, const FArgs<float>&, hipStream_t);
 template void launch_stage<double>(int, int, const FArgs<double>&, bool, bool, hipStream_t);
 template void launch_stage<float>(int, int, const FArgs<float>&, bool, bool, hipStream_t);
 template int pick_zc<double>(int, int, const FArgs<double>&, bool, bool);
 template int pick_zc<float>(int, int, const FArgs<float>&, bool, bool);
+template int pick_zc_emf<double>(int, int, const FArgs<double>&);
+template int pick_zc_emf<float>(int, int, const FArgs<float>&);
 
 }  // namespace ieq
 
@@ -488,6 +494,8 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
     }
     a.zc = pick_zc<R>(rr, ty, t, p.hall, hod);
   }
+  int zc_emf = env_int("IEQ_EMF_ZC", 0);
+  if (zc_emf < 1) zc_emf = pick_zc_emf<R>(rr, ty, a);
   for (long long step = 0; step < nsteps; ++step) {
     for (const StageSpec& st : plan) {
       a.P = nullptr;
@@ -495,7 +503,9 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
       if (st.kind == 3) {  // EMF of the field the next stage differentiates
         a.Y = st.Y == 0 ? b1 : F[st.Y];
         a.Eout = J;
-        emf<R>(rr, ty, a, s);
+        FArgs<R> ae = a;
+        ae.zc = zc_emf;
+        emf<R>(rr, ty, ae, s);
       } else if (st.kind == 0) {
         R* FF[3] = {b1, F[1], F[2]};
         a.Y = FF[st.Y];

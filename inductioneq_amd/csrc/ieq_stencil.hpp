@@ -218,5 +218,6 @@ template <typename R> void launch_emf(int rr, int ty, const FArgs<R>& a, hipStre
 template <typename R> void launch_stage(int rr, int ty, const FArgs<R>& a, bool hall, bool hod, hipStream_t s);
 // z-chunk that makes the stage kernel's tile grid one round of resident workgroups.
 template <typename R> int pick_zc(int rr, int ty, const FArgs<R>& a, bool hall, bool hod);
+template <typename R> int pick_zc_emf(int rr, int ty, const FArgs<R>& a);
 
 }  // namespace ieq
