@@ -4,14 +4,18 @@
 // (calc_div_divcleaning -> r; repeat { u = beta u + r; q = -Lap u; alpha = rr/(u.q);
 // phi += alpha u; r -= alpha q; rr = r.r } while rr > thr^2 and it < max; projection)
 // as two kernels per iteration with all scalars kept on the device:
-//   k_cg_lap     u' = beta*u + r (beta = rr/old, old = 1 at start), q = -Lap u' on the
+//   k_cg_lap     rr = r.r (sum of the previous update's partials), the stop test,
+//                u' = beta*u + r (beta = rr/old, old = 1 at start), q = -Lap u' on the
 //                BNODES interior, partial u'.q  (u' goes to the other direction register)
-//   k_cg_update  phi += alpha u', r -= alpha q, partial r.r
-// Dot products accumulate in FP64 whatever the field precision: every block writes
-// its partial and a one-workgroup kernel sums the partials in index order, so the
-// result is deterministic.  Once the stop test holds, every kernel
-// returns immediately, so any number of iterations can be enqueued without a host
-// round trip and the result equals stopping exactly at convergence.
+//   k_cg_update  alpha = rr / u'.q (sum of the Laplacian's partials), phi += alpha u',
+//                r -= alpha q, partial r.r
+// Dot products accumulate in FP64 whatever the field precision: every block writes its
+// partial, and every block of the next kernel sums all partials in the same fixed order, so
+// all blocks agree on alpha (beta) bit for bit and the result is deterministic.  rr of
+// iteration i is kept in slot i % 2 of the device state, so the block that records it never
+// overwrites the value the other blocks of the same launch read.  Once the stop test holds,
+// every kernel returns immediately, so any number of iterations can be enqueued without a
+// host round trip and the result equals stopping exactly at convergence.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -28,6 +32,7 @@ struct CgState {
   double rr, old, alpha, rr0;
   double tol2;       // stop when rr <= tol2 ...
   double rel2;       // ... or rr <= rel2 * rr0 (rel2 = 0: absolute test only)
+  double rrs[2];     // r.r of iteration i in slot i % 2 (slot 1 starts at 1: old of iteration 0)
   int it, max_it, done, pad;
 };
 
@@ -59,6 +64,16 @@ __device__ inline double sum_partials(const double* part, unsigned int n, double
   return block_sum(v, smem);
 }
 
+// The same sum, returned to every thread of the block (smem: at least 17 doubles).
+__device__ inline double sum_partials_all(const double* part, unsigned int n, double* smem) {
+  const double t = sum_partials(part, n, smem);
+  if (threadIdx.x == 0 && threadIdx.y == 0) smem[16] = t;
+  __syncthreads();
+  const double v = smem[16];
+  __syncthreads();
+  return v;
+}
+
 struct LapArgs {
   int nx, ny, nz, off;  // grid and BNODES
   int zc;               // planes per block
@@ -83,12 +98,26 @@ struct LapArgs {
 // the halo rows of y-neighbours are read through one L2.
 template <typename R, int HW, int TY, int PD>
 __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restrict__ u, const R* __restrict__ r,
-                                                    R* __restrict__ un, R* __restrict__ q, CgState* st, double* part) {
+                                                    R* __restrict__ un, R* __restrict__ q, CgState* st,
+                                                    const double* rr_part, unsigned int n_rr, double* part, int iter) {
   constexpr int LX = 64 + 2 * HW, LY = TY + 2 * HW, Q = 2 * HW + 1;
   __shared__ R s[2][LY][LX];
-  __shared__ double red[16];
+  __shared__ double red[17];
   if (st->done) return;
-  const R beta = (R)(st->rr / st->old);
+  // rr of this iteration from the previous update's partials (the initial residual's for
+  // iteration 0), the stop test, and beta = rr / old (clean_div_CG.m:45-48)
+  const double rr = sum_partials_all(rr_part, n_rr, red);
+  const double old = st->rrs[(iter + 1) & 1];
+  const bool go = rr > st->tol2 && (st->rel2 <= 0.0 || rr > st->rel2 * st->rr0) && iter < st->max_it;
+  if (threadIdx.x == 0 && threadIdx.y == 0 && blockIdx.x == 0) {
+    st->rr = rr;
+    st->old = old;
+    st->it = iter;
+    if (go) st->rrs[iter & 1] = rr;
+    else st->done = 1;
+  }
+  if (!go) return;
+  const R beta = (R)(rr / old);
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int gx = a.nx - 2 * a.off, gy = a.ny - 2 * a.off, gz = a.nz - 2 * a.off;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -245,36 +274,34 @@ __global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restri
   if (tx == 0 && ty == 0) part[tile] = bs;
 }
 
-// One workgroup: alpha = rr / u'.q from the block partials (summed in block order).  A
-// separate launch instead of a last-block finish: a device-scope fence in every block
-// writes back and invalidates the L2 of its XCD.
-__global__ __launch_bounds__(256) void k_cg_alpha(const double* part, unsigned int n, CgState* st) {
+// One workgroup, after the last enqueued iteration when the stop test never held: rr of the
+// final update and the iteration count (what the next Laplacian launch would record).
+__global__ __launch_bounds__(256) void k_cg_finish(const double* rr_part, unsigned int n, CgState* st, int iter) {
   __shared__ double red[16];
   if (st->done) return;
-  const double uq = sum_partials(part, n, red);
-  if (threadIdx.x == 0) st->alpha = st->rr / uq;
-}
-
-// One workgroup: rr = r.r, the stop test and the iteration count.
-__global__ __launch_bounds__(256) void k_cg_rr(const double* part, unsigned int n, CgState* st) {
-  __shared__ double red[16];
-  if (st->done) return;
-  const double rr = sum_partials(part, n, red);
+  const double rr = sum_partials(rr_part, n, red);
   if (threadIdx.x == 0) {
-    st->old = st->rr;
+    st->old = st->rrs[(iter + 1) & 1];
     st->rr = rr;
-    st->it += 1;
-    const bool go = rr > st->tol2 && (st->rel2 <= 0.0 || rr > st->rel2 * st->rr0) && st->it < st->max_it;
-    st->done = go ? 0 : 1;
+    st->it = iter;
+    st->done = 1;
   }
 }
 
+// Every block sums the Laplacian's u'.q partials in the same order (alpha = rr / u'.q).  The
+// reductions run inside the bulk kernels: a "last block finishes" scheme would need a
+// device-scope fence in every block, which on gfx950 writes back and invalidates the L2 of
+// its XCD.
 template <typename R>
 __global__ __launch_bounds__(kBlk) void k_cg_update(R* __restrict__ phi, const R* __restrict__ u, R* __restrict__ r,
-                                                    const R* __restrict__ q, long long n, CgState* st, double* part) {
-  __shared__ double red[16];
+                                                    const R* __restrict__ q, long long n, CgState* st,
+                                                    const double* uq_part, unsigned int n_uq, double* part, int iter) {
+  __shared__ double red[17];
   if (st->done) return;
-  const R alpha = (R)st->alpha;
+  const double uq = sum_partials_all(uq_part, n_uq, red);
+  const double alpha_d = st->rrs[iter & 1] / uq;
+  if (threadIdx.x == 0 && blockIdx.x == 0) st->alpha = alpha_d;
+  const R alpha = (R)alpha_d;
   double acc = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     phi[i] = phi[i] + alpha * u[i];
@@ -306,6 +333,8 @@ __global__ __launch_bounds__(256) void k_cg_init_state(const double* part, unsig
     st->rr = rr;
     st->rr0 = rr;
     st->old = 1.0;  // clean_div_CG.m:43
+    st->rrs[0] = 0.0;
+    st->rrs[1] = 1.0;  // old of iteration 0
     st->alpha = 0.0;
     st->it = 0;
     st->max_it = max_it;
@@ -321,10 +350,11 @@ __global__ __launch_bounds__(256) void k_cg_init_state(const double* part, unsig
 constexpr int kLapPD = IEQ_LAP_PD;  // planes of the Laplacian's z-march in flight per thread
 
 template <typename R, int HW, int TY>
-void launch_lap(const LapArgs& a, const R* u, const R* r, R* un, R* q, CgState* st, double* part, hipStream_t s) {
+void launch_lap(const LapArgs& a, const R* u, const R* r, R* un, R* q, CgState* st, const double* rr_part, unsigned int n_rr,
+                double* part, int iter, hipStream_t s) {
   const int gx = a.nx - 2 * a.off, gy = a.ny - 2 * a.off, gz = a.nz - 2 * a.off;
   const int grid = ((gx + 63) / 64) * ((gy + TY - 1) / TY) * ((gz + a.zc - 1) / a.zc);
-  k_cg_lap<R, HW, TY, kLapPD><<<grid, dim3(64, TY), 0, s>>>(a, u, r, un, q, st, part);
+  k_cg_lap<R, HW, TY, kLapPD><<<grid, dim3(64, TY), 0, s>>>(a, u, r, un, q, st, rr_part, n_rr, part, iter);
 }
 
 // Planes per z-chunk: the smallest chunk whose grid fits in one round of resident
@@ -379,7 +409,7 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
   const unsigned int lap_blocks = (unsigned)(((gx + 63) / 64) * ((gy + TY - 1) / TY) * ((gz + zc - 1) / zc));
   const unsigned int vec_blocks = 1024;
   const size_t fbytes = ((size_t)info.stride * sizeof(R) + 255) / 256 * 256;
-  const size_t part_bytes = (size_t)std::max(lap_blocks, vec_blocks) * sizeof(double);
+  const size_t part_bytes = (size_t)(lap_blocks + vec_blocks) * sizeof(double);  // u'.q partials, then r.r partials
   const size_t tab_bytes = (size_t)(2 * nbl + 1) * wl * sizeof(double);
   const size_t need = 4 * fbytes + ((part_bytes + 255) / 256 * 256) + 256 + tab_bytes;
   if (!*work) *work = new CgWork();
@@ -395,7 +425,8 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
   R* r = (R*)base;
   R* ub[2] = {(R*)(base + fbytes), (R*)(base + 3 * fbytes)};  // direction registers, alternating
   R* q = (R*)(base + 2 * fbytes);
-  double* part = (double*)(base + 4 * fbytes);
+  double* part_uq = (double*)(base + 4 * fbytes);
+  double* part_rr = part_uq + lap_blocks;
   CgState* st = (CgState*)(base + 4 * fbytes + (part_bytes + 255) / 256 * 256);
   double* tab = (double*)((char*)st + 256);
   if (hipMemsetAsync(base, 0, 4 * fbytes, s) != hipSuccess || hipMemsetAsync(phi, 0, fbytes, s) != hipSuccess ||
@@ -410,8 +441,8 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
   // r = div B on the interior (calc_div_divcleaning, kernel_div_cleaning.cl:60-70)
   const uint32_t g_dc[3] = {(uint32_t)(p.nx - 2 * p.bnodes), (uint32_t)(p.ny - 2 * p.bnodes), (uint32_t)(p.nz - 2 * p.bnodes)};
   gen_div<R>(p, info.opset, s, g_dc, b, r, p.bnodes);
-  k_cg_init<R><<<vec_blocks, kBlk, 0, s>>>(r, n, part);
-  k_cg_init_state<<<1, 256, 0, s>>>(part, vec_blocks, st, abs_thr * abs_thr, rel_tol * rel_tol, max_it);
+  k_cg_init<R><<<vec_blocks, kBlk, 0, s>>>(r, n, part_rr);
+  k_cg_init_state<<<1, 256, 0, s>>>(part_rr, vec_blocks, st, abs_thr * abs_thr, rel_tol * rel_tol, max_it);
 
   LapArgs la;
   la.nx = p.nx;
@@ -436,23 +467,22 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
       const R* u = ub[it & 1];
       R* un = ub[(it + 1) & 1];
       switch (hw) {
-        case 1: launch_lap<R, 1, 8>(la, u, r, un, q, st, part, s); break;
-        case 2: launch_lap<R, 2, 8>(la, u, r, un, q, st, part, s); break;
-        case 3: launch_lap<R, 3, 8>(la, u, r, un, q, st, part, s); break;
-        case 4: launch_lap<R, 4, 8>(la, u, r, un, q, st, part, s); break;
-        case 5: launch_lap<R, 5, 8>(la, u, r, un, q, st, part, s); break;
-        case 6: launch_lap<R, 6, 8>(la, u, r, un, q, st, part, s); break;
-        default: launch_lap<R, 7, 8>(la, u, r, un, q, st, part, s); break;
+        case 1: launch_lap<R, 1, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 2: launch_lap<R, 2, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 3: launch_lap<R, 3, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 4: launch_lap<R, 4, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 5: launch_lap<R, 5, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 6: launch_lap<R, 6, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        default: launch_lap<R, 7, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
       }
-      k_cg_alpha<<<1, 256, 0, s>>>(part, lap_blocks, st);
-      k_cg_update<R><<<vec_blocks, kBlk, 0, s>>>(phi, un, r, q, n, st, part);
-      k_cg_rr<<<1, 256, 0, s>>>(part, vec_blocks, st);
+      k_cg_update<R><<<vec_blocks, kBlk, 0, s>>>(phi, un, r, q, n, st, part_uq, lap_blocks, part_rr, it);
     }
     launched += k;
     if (hipMemcpyAsync(&hst, st, sizeof(CgState), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
       return err = "CG state read failed", -4;
     if (hst.done) break;
   }
+  k_cg_finish<<<1, 256, 0, s>>>(part_rr, vec_blocks, st, launched);  // no-op once the stop test held
   (void)hipEventRecord(e1, s);
   (void)hipEventSynchronize(e1);
   float ms = 0.f;
