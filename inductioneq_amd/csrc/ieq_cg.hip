@@ -348,6 +348,10 @@ __global__ __launch_bounds__(256) void k_cg_init_state(const double* part, unsig
 #define IEQ_LAP_PD 2
 #endif
 constexpr int kLapPD = IEQ_LAP_PD;  // planes of the Laplacian's z-march in flight per thread
+#ifndef IEQ_LAP_TY
+#define IEQ_LAP_TY 8
+#endif
+constexpr int kLapTY = IEQ_LAP_TY;  // rows of a Laplacian tile (64 x kLapTY threads)
 
 template <typename R, int HW, int TY>
 void launch_lap(const LapArgs& a, const R* u, const R* r, R* un, R* q, CgState* st, const double* rr_part, unsigned int n_rr,
@@ -394,17 +398,17 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
   const int hw = (wl - 1) / 2;
   if (hw < 1 || hw > 7 || info.periodic || 2 * nbl + 1 > kLapRows || !offsets_fit(p.nx, p.ny, p.nz, sizeof(R)))
     return 1;  // not covered by the fused loop
-  const int TY = 8;
+  const int TY = kLapTY;
   const int gx = p.nx - 2 * p.bnodes, gy = p.ny - 2 * p.bnodes, gz = p.nz - 2 * p.bnodes;
   int zc = 8;
   switch (hw) {
-    case 1: zc = lap_zc<R, 1, 8>(gx, gy, gz); break;
-    case 2: zc = lap_zc<R, 2, 8>(gx, gy, gz); break;
-    case 3: zc = lap_zc<R, 3, 8>(gx, gy, gz); break;
-    case 4: zc = lap_zc<R, 4, 8>(gx, gy, gz); break;
-    case 5: zc = lap_zc<R, 5, 8>(gx, gy, gz); break;
-    case 6: zc = lap_zc<R, 6, 8>(gx, gy, gz); break;
-    default: zc = lap_zc<R, 7, 8>(gx, gy, gz); break;
+    case 1: zc = lap_zc<R, 1, kLapTY>(gx, gy, gz); break;
+    case 2: zc = lap_zc<R, 2, kLapTY>(gx, gy, gz); break;
+    case 3: zc = lap_zc<R, 3, kLapTY>(gx, gy, gz); break;
+    case 4: zc = lap_zc<R, 4, kLapTY>(gx, gy, gz); break;
+    case 5: zc = lap_zc<R, 5, kLapTY>(gx, gy, gz); break;
+    case 6: zc = lap_zc<R, 6, kLapTY>(gx, gy, gz); break;
+    default: zc = lap_zc<R, 7, kLapTY>(gx, gy, gz); break;
   }
   const unsigned int lap_blocks = (unsigned)(((gx + 63) / 64) * ((gy + TY - 1) / TY) * ((gz + zc - 1) / zc));
   const unsigned int vec_blocks = 1024;
@@ -467,13 +471,13 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
       const R* u = ub[it & 1];
       R* un = ub[(it + 1) & 1];
       switch (hw) {
-        case 1: launch_lap<R, 1, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        case 2: launch_lap<R, 2, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        case 3: launch_lap<R, 3, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        case 4: launch_lap<R, 4, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        case 5: launch_lap<R, 5, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        case 6: launch_lap<R, 6, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        default: launch_lap<R, 7, 8>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 1: launch_lap<R, 1, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 2: launch_lap<R, 2, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 3: launch_lap<R, 3, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 4: launch_lap<R, 4, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 5: launch_lap<R, 5, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 6: launch_lap<R, 6, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        default: launch_lap<R, 7, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
       }
       k_cg_update<R><<<vec_blocks, kBlk, 0, s>>>(phi, un, r, q, n, st, part_uq, lap_blocks, part_rr, it);
     }
