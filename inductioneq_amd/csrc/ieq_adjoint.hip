@@ -58,7 +58,8 @@ __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
   const unsigned colb = (unsigned)(wrap(x0 + tx, a.nx) + a.nx * wrap(y0 + ty, a.ny)) * RB;
   const unsigned plb = (unsigned)(a.nx * a.ny) * RB;
   const int zend = min(z0 + a.zc, a.nz);
-  PlaneStage<R, RR, TY, 3> hy, hl;
+  PlaneStage<R, RR, TY, 3> hy;
+  PlaneStage<R, RR, TY, 3, 0b110, 0b101> hl;  // L: x halo L_y, L_z;  y halo L_x, L_z
   hy.init(tx, ty, x0, y0, a.nx, a.ny);
   hl.init(tx, ty, x0, y0, a.nx, a.ny);
   const Comps<R, 3> Yc = comps<R, 3>(a.Y, S), Lc = comps<R, 3>(a.L, S);
@@ -285,7 +286,6 @@ template <typename R> FArgs<R> base_args(const Adjoint& A, const StencilCoef<R>&
   for (int d = 0; d < 3; ++d) {
     a.inv_h[d] = (R)(1.0 / h[d]);
     a.two_h[d] = (R)(2.0 / h[d]);
-    a.h[d] = (R)h[d];
   }
   a.rho = (R)(1.0 / A.c_hall);
   for (int d = 0; d < 3; ++d) a.u[d] = (R)A.u[d];

@@ -58,7 +58,6 @@ template <typename R> FArgs<R> member_base(const Ensemble& e, const ieq_member& 
   for (int d = 0; d < 3; ++d) {
     a.inv_h[d] = (R)(1.0 / h[d]);
     a.two_h[d] = (R)(2.0 / h[d]);
-    a.h[d] = (R)h[d];
   }
   a.rho = (R)mb.rho;
   a.u[0] = (R)mb.ux;

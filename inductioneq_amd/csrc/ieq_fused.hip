@@ -459,7 +459,6 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
   for (int d = 0; d < 3; ++d) {
     a.inv_h[d] = (R)(1.0 / h[d]);
     a.two_h[d] = (R)(2.0 / h[d]);
-    a.h[d] = (R)h[d];
   }
   a.rho = (R)mn[3];
   a.u[0] = (R)mn[0];

@@ -48,7 +48,7 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
   const int zend = min(z0 + a.zc, a.nz);
   const Comps<R, 3> B = comps<R, 3>(a.Y, a.S);
   const R irho = R(1) / a.rho;
-  PlaneStage<R, RR, TY, 3> hs;
+  PlaneStage<R, RR, TY, 3, 0b110, 0b101> hs;  // x halo: B_y, B_z;  y halo: B_x, B_z
   hs.init(tx, ty, x0, y0, a.nx, a.ny);
 
   R q[3][Q];        // own column: planes it-RR .. it+RR
@@ -200,7 +200,7 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
   const R* Qp = a.Q ? a.Q : a.Y;
   const R* Gp = a.G ? a.G : a.Y;
   PlaneStage<R, RR, TY, 3> hsb;
-  PlaneStage<R, RR, TY, 3> hse;
+  PlaneStage<R, RR, TY, 3, 0b110, 0b101> hse;  // x halo: E_y, E_z;  y halo: E_x, E_z
   hsb.init(tx, ty, x0, y0, a.nx, a.ny);
   if constexpr (HALL) hse.init(tx, ty, x0, y0, a.nx, a.ny);
 

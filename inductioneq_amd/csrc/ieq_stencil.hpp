@@ -19,12 +19,10 @@ template <typename R> struct StencilCoef {
 
 template <typename R> struct FArgs {
   int nx, ny, nz, zc;
-  int nbuf;  // LDS plane buffers of the single-grid kernels: 2 (pipelined) or 1 (half the LDS)
   long long S;
   StencilCoef<R> c;
-  R inv_h[3];  // (R)(1/dx_d): Hall_volume scaling
+  R inv_h[3];  // (R)(1/dx_d): Hall_volume and curl scaling
   R two_h[3];  // (R)(2/dx_d): convec_volume scaling
-  R h[3];      // (R)dx_d: derivative divisor of curl
   R rho;       // uniform density (J = curl B / rho)
   R u[3];      // uniform velocity
   int conv;    // velocity non-zero: evaluate the convection term
@@ -127,7 +125,9 @@ template <typename R, int NC> __device__ __forceinline__ Comps<R, NC> comps(cons
 // Per-thread staging of one xy-plane of an NC-component field into an LDS tile with a
 // halo of RR cells.  The thread owns its column (x0+tx, y0+ty) and, at the tile edges,
 // up to one x-halo cell and up to two y-halo cells (all coordinates wrap periodically).
-template <typename R, int RR, int TY, int NC> struct PlaneStage {
+// XM / YM select the components whose x / y halo is staged (bit c = component c): a
+// component that is only differentiated along y needs no x halo, and vice versa.
+template <typename R, int RR, int TY, int NC, int XM = (1 << NC) - 1, int YM = (1 << NC) - 1> struct PlaneStage {
   static constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR;
   // With TY >= 2*RR a thread stages at most one y-halo cell (top rows or bottom rows).
   static constexpr bool ONE_Y = TY >= 2 * RR;
@@ -155,31 +155,38 @@ template <typename R, int RR, int TY, int NC> struct PlaneStage {
   __device__ void load(const Comps<R, NC>& f, unsigned pz) {
     if (hx)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) vx[c] = ldo(f.c[c], gx + pz);
+      for (int c = 0; c < NC; ++c)
+        if ((XM >> c) & 1) vx[c] = ldo(f.c[c], gx + pz);
     if (ht)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) vt[c] = ldo(f.c[c], gt + pz);
+      for (int c = 0; c < NC; ++c)
+        if ((YM >> c) & 1) vt[c] = ldo(f.c[c], gt + pz);
     if constexpr (!ONE_Y) {
       if (hb)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) vb[c] = ldo(f.c[c], gb + pz);
+        for (int c = 0; c < NC; ++c)
+          if ((YM >> c) & 1) vb[c] = ldo(f.c[c], gb + pz);
     }
   }
   __device__ void store(R (*sm)[LY][LX], int tx, int ty) {
     if (hx)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) sm[c][ty + RR][lcx] = vx[c];
+      for (int c = 0; c < NC; ++c)
+        if ((XM >> c) & 1) sm[c][ty + RR][lcx] = vx[c];
     if constexpr (ONE_Y) {
       if (ht)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) sm[c][lry][tx + RR] = vt[c];
+        for (int c = 0; c < NC; ++c)
+          if ((YM >> c) & 1) sm[c][lry][tx + RR] = vt[c];
     } else {
       if (ht)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) sm[c][ty][tx + RR] = vt[c];
+        for (int c = 0; c < NC; ++c)
+          if ((YM >> c) & 1) sm[c][ty][tx + RR] = vt[c];
       if (hb)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) sm[c][ty + 2 * RR][tx + RR] = vb[c];
+        for (int c = 0; c < NC; ++c)
+          if ((YM >> c) & 1) sm[c][ty + 2 * RR][tx + RR] = vb[c];
     }
   }
 };
