@@ -295,8 +295,10 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
             R h = R(0);
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
+              // (the centre value is the queue head, so only the 2*RR neighbours come from LDS)
 #pragma unroll
-              for (int k = 0; k < Q; ++k) v[k] = d == 0 ? sB[buf][c][ty + RR][tx + k] : sB[buf][c][ty + k][tx + RR];
+              for (int k = 0; k < Q; ++k)
+                v[k] = k == RR ? qb[c][Q - 1] : d == 0 ? sB[buf][c][ty + RR][tx + k] : sB[buf][c][ty + k][tx + RR];
               if constexpr (CONV) {
                 const R der = deriv<R, RR>(a.c, v);
                 const R f = a.two_h[d] * R(0.5);
