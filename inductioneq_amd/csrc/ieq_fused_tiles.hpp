@@ -55,8 +55,11 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
   R cs[PD][3];      // column loads in flight: plane p in slot (p - z0) % PD
   R jr[3][RR + 1];  // x/y parts of curl_D B for planes it .. it+RR: (d_y B_z, -d_x B_z, d_x B_y - d_y B_x)
   const int it0 = z0 - RR;
+  // Loads issued for planes past the last one the chunk needs (the prefetch overshoot of the
+  // final iterations) re-read that last plane, a cache hit, instead of fetching unused planes.
+  const int zlast = zend - 1 + RR;
   auto col = [&](R* dst, int pl) {
-    const unsigned o = colb + (unsigned)wrap(pl, a.nz) * plb;
+    const unsigned o = colb + (unsigned)wrap(min(pl, zlast), a.nz) * plb;
 #pragma unroll
     for (int c = 0; c < 3; ++c) dst[c] = ldo(B.c[c], o);
   };
@@ -210,8 +213,10 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
   R pw[PD][NPW > 0 ? NPW : 1];  // pointwise loads in flight: output plane o in slot (o - it0) % PD
   R rx[3][RR + 1];             // x/y terms of the RHS for planes it .. it+RR
   const int it0 = z0 - RR;
+  // Loads past the last plane the chunk needs re-read that plane (a cache hit), as in emf_tile.
+  const int zlast = zend - 1 + RR;
   auto colld = [&](R* d, int pl) {
-    const unsigned o = colb + (unsigned)wrap(pl, a.nz) * plb;
+    const unsigned o = colb + (unsigned)wrap(min(pl, zlast), a.nz) * plb;
 #pragma unroll
     for (int c = 0; c < 3; ++c) d[c] = ldo(B.c[c], o);
     if constexpr (HALL) {
@@ -221,7 +226,7 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
   };
   auto ptld = [&](R* d, int pl) {
     if constexpr (NPW == 0) return;
-    const unsigned o = colb + (unsigned)wrap(pl, a.nz) * plb;
+    const unsigned o = colb + (unsigned)wrap(min(pl, zend - 1), a.nz) * plb;
 #pragma unroll
     for (int c = 0; c < 3; ++c) d[c] = ldo(Pp + c * S, o);
     if constexpr (MODE == 0) {
