@@ -257,7 +257,8 @@ struct Adjoint {
   double* partial = nullptr;
   double* acc = nullptr;
   int nblk = 0;
-  int ty = 8, zc = 8;  // tile rows and z-chunk of every launch (chosen at creation)
+  int ty = 8, zc = 8;  // tile rows of k_adj_pre (64-wide tiles) and z-chunk of every launch
+  int shape = 8;       // tile shape code of the stage and EMF launches (as in ieq_step)
   int zc_emf = 8;      // z-chunk of the EMF launches
   Binomial bin;
   double c_hall = 1.0, u[3] = {0, 0, 0};
@@ -302,7 +303,7 @@ template <typename R> StencilCoef<R> coefs(const Adjoint& A) {
 
 template <typename R> void emf_launch(const Adjoint& A, FArgs<R> a) {
   a.zc = A.zc_emf;
-  launch_emf<R>(A.rr, A.ty, a, A.s);
+  launch_emf<R>(A.rr, A.shape, a, A.s);
 }
 
 // One forward SSPRK33 step from b1 with stage registers b2, b3 (and E); the last stage writes
@@ -330,7 +331,7 @@ template <typename R> void fwd_step_to(Adjoint& A, R* b1, R* b2, R* b3, R* out) 
     a.cP = (R)cPs[k];
     a.cQ = (R)cQs[k];
     a.cD = (R)(cDs[k] * A.p.dt);
-    launch_stage<R>(A.rr, A.ty, a, hall, hod, A.s);
+    launch_stage<R>(A.rr, A.shape, a, hall, hod, A.s);
   }
   ++A.fwd_steps;
 }
@@ -393,7 +394,7 @@ template <typename R> void adj_stage(Adjoint& A, const R* L, R* out, const R* P,
     a.E = nullptr;
     a.G = nullptr;
   }
-  launch_stage<R>(A.rr, A.ty, a, A.p.hall != 0, A.p.diss == 1, A.s);
+  launch_stage<R>(A.rr, A.shape, a, A.p.hall != 0, A.p.diss == 1, A.s);
 }
 
 // Reverse one SSPRK33 step given its stage inputs b1, b2, b3: lam <- step^T lam, parameter partials.
@@ -428,14 +429,14 @@ template <typename R> void adj_step(Adjoint& A, const R* b1) {
   }
   a.Y = b1; a.E = (R*)A.E; a.mode = 0; a.out = b2; a.P = b1; a.Q = nullptr; a.cP = R(1); a.cQ = R(0);
   a.cD = (R)(1.0 * A.p.dt);
-  launch_stage<R>(A.rr, A.ty, a, hall, hod, A.s);
+  launch_stage<R>(A.rr, A.shape, a, hall, hod, A.s);
   if (hall) {
     a.Y = b2;
     a.Eout = (R*)A.E;
     emf_launch<R>(A, a);
   }
   a.Y = b2; a.E = (R*)A.E; a.out = b3; a.P = b1; a.Q = b2; a.cP = (R)0.75; a.cQ = (R)0.25; a.cD = (R)(0.25 * A.p.dt);
-  launch_stage<R>(A.rr, A.ty, a, hall, hod, A.s);
+  launch_stage<R>(A.rr, A.shape, a, hall, hod, A.s);
   adj_from<R>(A, b1, b2, b3);
 }
 
@@ -593,6 +594,7 @@ int adjoint_create(const Params& p, const ieq_program_info& info, long long nste
   // tiles as in ieq_step: FP64 4 rows, FP32 8; the z-chunk that fills one round of resident
   // workgroups of the forward stage kernel (SSPRK33 stage 2 form, convection on)
   A->ty = info.real_bytes == 8 ? 4 : 8;
+  A->shape = info.real_bytes == 8 ? 32 : 8;
   {
     int fake[3];  // distinct addresses standing for b1, b2, E: only compared, never read
     if (info.real_bytes == 8) {
@@ -603,8 +605,8 @@ int adjoint_create(const Params& p, const ieq_program_info& info, long long nste
       t.Y = t.Q = (const double*)&fake[1];
       t.P = (const double*)&fake[0];
       t.E = (const double*)&fake[2];
-      A->zc = pick_zc<double>(A->rr, A->ty, t, p.hall != 0, p.diss == 1);
-      A->zc_emf = pick_zc_emf<double>(A->rr, A->ty, t);
+      A->zc = pick_zc<double>(A->rr, A->shape, t, p.hall != 0, p.diss == 1);
+      A->zc_emf = pick_zc_emf<double>(A->rr, A->shape, t);
     } else {
       StencilCoef<float> cc;
       central_coefs<float>(info.opset, p.ho_factor, cc);
@@ -613,8 +615,8 @@ int adjoint_create(const Params& p, const ieq_program_info& info, long long nste
       t.Y = t.Q = (const float*)&fake[1];
       t.P = (const float*)&fake[0];
       t.E = (const float*)&fake[2];
-      A->zc = pick_zc<float>(A->rr, A->ty, t, p.hall != 0, p.diss == 1);
-      A->zc_emf = pick_zc_emf<float>(A->rr, A->ty, t);
+      A->zc = pick_zc<float>(A->rr, A->shape, t, p.hall != 0, p.diss == 1);
+      A->zc_emf = pick_zc_emf<float>(A->rr, A->shape, t);
     }
   }
   A->nblk = ((p.nx + 63) / 64) * ((p.ny + A->ty - 1) / A->ty) * ((p.nz + A->zc - 1) / A->zc);

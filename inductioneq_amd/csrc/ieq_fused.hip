@@ -151,21 +151,22 @@ template <typename R> __global__ void k_advance_time(R* time, R dt, long long ns
   }
 }
 
-template <typename R, int RR, int TY, int NBUF> const void* emf_kernel_ty() {
-  return reinterpret_cast<const void*>(&k_emf<R, RR, TY, NBUF>);
+template <typename R, int RR, int TY, int NBUF, int TX = 64> const void* emf_kernel_ty() {
+  return reinterpret_cast<const void*>(&k_emf<R, RR, TY, NBUF, TX>);
 }
 
-template <typename R, int RR, int TY, bool ADJ, int NBUF, int MODE> const void* stage_kernel_hdc(bool hall, bool hod, bool conv) {
+template <typename R, int RR, int TY, bool ADJ, int NBUF, int MODE, int TX>
+const void* stage_kernel_hdc(bool hall, bool hod, bool conv) {
   if (conv) {
-    if (hall && hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, true, true, ADJ, NBUF, MODE>);
-    if (hall) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, false, true, ADJ, NBUF, MODE>);
-    if (hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, true, true, ADJ, NBUF, MODE>);
-    return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, false, true, ADJ, NBUF, MODE>);
+    if (hall && hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, true, true, ADJ, NBUF, MODE, TX>);
+    if (hall) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, false, true, ADJ, NBUF, MODE, TX>);
+    if (hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, true, true, ADJ, NBUF, MODE, TX>);
+    return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, false, true, ADJ, NBUF, MODE, TX>);
   }
-  if (hall && hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, true, false, ADJ, NBUF, MODE>);
-  if (hall) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, false, false, ADJ, NBUF, MODE>);
-  if (hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, true, false, ADJ, NBUF, MODE>);
-  return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, false, false, ADJ, NBUF, MODE>);
+  if (hall && hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, true, false, ADJ, NBUF, MODE, TX>);
+  if (hall) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, true, false, false, ADJ, NBUF, MODE, TX>);
+  if (hod) return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, true, false, ADJ, NBUF, MODE, TX>);
+  return reinterpret_cast<const void*>(&k_stage<R, RR, TY, false, false, false, ADJ, NBUF, MODE, TX>);
 }
 
 // LDS plane buffers of the single-grid stage kernel: 2 (one barrier per plane) or 1 (half the
@@ -176,37 +177,47 @@ template <typename R, int RR, int TY, bool ADJ, int NBUF, int MODE> const void* 
 // Stage kernel for the arguments: the transposed stages of the adjoint sweeps (G, src_adj or
 // cY set) use the three-register form with convection always evaluated; otherwise the form
 // follows stage_mode.
-template <typename R, int RR, int TY> const void* stage_kernel_ty(const FArgs<R>& a, bool hall, bool hod) {
-  if (a.G || a.src_adj || a.cY != R(0)) return stage_kernel_hdc<R, RR, TY, true, IEQ_STAGE_NBUF, 0>(hall, hod, true);
+template <typename R, int RR, int TY, int TX = 64> const void* stage_kernel_ty(const FArgs<R>& a, bool hall, bool hod) {
+  if (a.G || a.src_adj || a.cY != R(0)) return stage_kernel_hdc<R, RR, TY, true, IEQ_STAGE_NBUF, 0, TX>(hall, hod, true);
   switch (stage_mode(a.mode == 1, a.P, a.Q, a.Y)) {
-    case 1: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 1>(hall, hod, a.conv);
-    case 2: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 2>(hall, hod, a.conv);
-    case 3: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 3>(hall, hod, a.conv);
-    default: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 0>(hall, hod, a.conv);
+    case 1: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 1, TX>(hall, hod, a.conv);
+    case 2: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 2, TX>(hall, hod, a.conv);
+    case 3: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 3, TX>(hall, hod, a.conv);
+    default: return stage_kernel_hdc<R, RR, TY, false, IEQ_STAGE_NBUF, 0, TX>(hall, hod, a.conv);
   }
 }
 
-template <typename R> const void* emf_kernel(int rr, int ty) {
-  if (ty == 8) return rr == 1 ? emf_kernel_ty<R, 1, 8, 2>() : rr == 2 ? emf_kernel_ty<R, 2, 8, 2>() : emf_kernel_ty<R, 3, 8, 2>();
+// Tile shapes of the single-grid launches, by code: 4 -> 64 x 4, 8 -> 64 x 8, 32 -> 32 x 8 (a
+// narrower tile whose y halo is a smaller share of the loads).
+constexpr int shape_tx(int code) { return code == 32 ? 32 : 64; }
+constexpr int shape_ty(int code) { return code == 32 ? 8 : code; }
+
+template <typename R> const void* emf_kernel(int rr, int code) {
+  if (code == 32) return rr == 1 ? emf_kernel_ty<R, 1, 8, 2, 32>() : rr == 2 ? emf_kernel_ty<R, 2, 8, 2, 32>() : emf_kernel_ty<R, 3, 8, 2, 32>();
+  if (code == 8) return rr == 1 ? emf_kernel_ty<R, 1, 8, 2>() : rr == 2 ? emf_kernel_ty<R, 2, 8, 2>() : emf_kernel_ty<R, 3, 8, 2>();
   return rr == 1 ? emf_kernel_ty<R, 1, 4, 2>() : rr == 2 ? emf_kernel_ty<R, 2, 4, 2>() : emf_kernel_ty<R, 3, 4, 2>();
 }
-template <typename R> const void* stage_kernel(int rr, int ty, const FArgs<R>& a, bool hall, bool hod) {
-  if (ty == 8)
+template <typename R> const void* stage_kernel(int rr, int code, const FArgs<R>& a, bool hall, bool hod) {
+  if (code == 32)
+    return rr == 1 ? stage_kernel_ty<R, 1, 8, 32>(a, hall, hod) : rr == 2 ? stage_kernel_ty<R, 2, 8, 32>(a, hall, hod)
+                                                                          : stage_kernel_ty<R, 3, 8, 32>(a, hall, hod);
+  if (code == 8)
     return rr == 1 ? stage_kernel_ty<R, 1, 8>(a, hall, hod) : rr == 2 ? stage_kernel_ty<R, 2, 8>(a, hall, hod) : stage_kernel_ty<R, 3, 8>(a, hall, hod);
   return rr == 1 ? stage_kernel_ty<R, 1, 4>(a, hall, hod) : rr == 2 ? stage_kernel_ty<R, 2, 4>(a, hall, hod) : stage_kernel_ty<R, 3, 4>(a, hall, hod);
 }
 
-template <typename R> int tile_grid(const FArgs<R>& a, int ty) {
-  return ((a.nx + 63) / 64) * ((a.ny + ty - 1) / ty) * ((a.nz + a.zc - 1) / a.zc);
+template <typename R> int tile_grid(const FArgs<R>& a, int code) {
+  const int tx = shape_tx(code), ty = shape_ty(code);
+  return ((a.nx + tx - 1) / tx) * ((a.ny + ty - 1) / ty) * ((a.nz + a.zc - 1) / a.zc);
 }
-template <typename R> void launch_k(const void* k, int ty, const FArgs<R>& a, hipStream_t s) {
+template <typename R> void launch_k(const void* k, int code, const FArgs<R>& a, hipStream_t s) {
   FArgs<R> arg = a;
   void* kargs[1] = {(void*)&arg};
-  (void)hipLaunchKernel(k, dim3(tile_grid(a, ty)), dim3(64, ty), kargs, 0, s);
+  (void)hipLaunchKernel(k, dim3(tile_grid(a, code)), dim3(shape_tx(code), shape_ty(code)), kargs, 0, s);
 }
-template <typename R> void emf(int rr, int ty, const FArgs<R>& a, hipStream_t s) { launch_k(emf_kernel<R>(rr, ty), ty, a, s); }
-template <typename R> void stage(int rr, int ty, const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
-  launch_k(stage_kernel<R>(rr, ty, a, hall, hod), ty, a, s);
+template <typename R> void emf(int rr, int code, const FArgs<R>& a, hipStream_t s) { launch_k(emf_kernel<R>(rr, code), code, a, s); }
+template <typename R> void stage(int rr, int code, const FArgs<R>& a, bool hall, bool hod, hipStream_t s) {
+  launch_k(stage_kernel<R>(rr, code, a, hall, hod), code, a, s);
 }
 
 int env_int(const char* k, int dflt) {
@@ -231,12 +242,13 @@ template <typename R> void launch_stage(int rr, int ty, const FArgs<R>& a, bool 
 // z-chunk for a grid: the smallest chunk (at least 4 planes) whose tile grid fits one round of
 // resident workgroups of the stage kernel the arguments select, so no partial second round
 // and the shortest possible z lead-in for the available parallelism.
-static int zc_for(const void* kernel, int ty, int nx, int ny, int nz) {
+static int zc_for(const void* kernel, int code, int nx, int ny, int nz) {
+  const int tx = shape_tx(code), ty = shape_ty(code);
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * ty, 0) != hipSuccess || cus <= 0 || per_cu <= 0)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, tx * ty, 0) != hipSuccess || cus <= 0 || per_cu <= 0)
     return 8;
-  const long long xy = (long long)((nx + 63) / 64) * ((ny + ty - 1) / ty);
+  const long long xy = (long long)((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
   const long long chunks = std::max(1LL, (long long)cus * per_cu / std::max(1LL, xy));
   return std::max(4, (int)((nz + chunks - 1) / chunks));
 }
@@ -449,10 +461,12 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
   a.nx = p.nx;
   a.ny = p.ny;
   a.nz = p.nz;
-  // Tile rows measured at 128^3: FP64 4 (2 waves per SIMD, so smaller workgroups), FP32 8.
-  // The z-chunk is set below from the stage kernel's occupancy (16 and 8 planes at 128^3).
+  // Tile shape measured at 128^3: FP64 32 x 8 (2 waves per SIMD, so small workgroups; the
+  // narrow tile keeps the y halo a smaller share of the loads than 64 x 4), FP32 64 x 8.
+  // The z-chunk is set below from the stage kernel's occupancy.
   const bool f64 = sizeof(R) == 8;
-  const int ty = env_int("IEQ_FUSED_TY", f64 ? 4 : 8) == 4 ? 4 : 8;
+  const int ty_env = env_int("IEQ_FUSED_TY", f64 ? 32 : 8);  // tile shape code: 4, 8 or 32 (32 x 8)
+  const int ty = ty_env == 4 || ty_env == 32 ? ty_env : 8;
   a.S = info.stride;
   a.c = coef;
   const double h[3] = {p.dx, p.dy, p.dz};

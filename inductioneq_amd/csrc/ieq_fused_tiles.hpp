@@ -37,8 +37,8 @@ template <typename R, int RR> constexpr int march_pd() { return (sizeof(R) == 8 
 // complete J(it).  The column value and the halo of a plane are therefore read in the same
 // iteration, which is also when the neighbouring tiles (dispatched together to the same
 // XCD) read those rows as their own column values, so the halo rows are L2 hits.
-template <typename R, int RR, int TY, int NBUF = 2>
-__device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int z0, R (*sB)[3][TY + 2 * RR][64 + 2 * RR]) {
+template <typename R, int RR, int TY, int NBUF = 2, int TX = 64>
+__device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int z0, R (*sB)[3][TY + 2 * RR][TX + 2 * RR]) {
   constexpr int Q = 2 * RR + 1, PD = march_pd<R, RR>();
   const int tx = threadIdx.x, ty = threadIdx.y;
   const bool active = x0 + tx < a.nx && y0 + ty < a.ny;
@@ -48,7 +48,7 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
   const int zend = min(z0 + a.zc, a.nz);
   const Comps<R, 3> B = comps<R, 3>(a.Y, a.S);
   const R irho = R(1) / a.rho;
-  PlaneStage<R, RR, TY, 3, 0b110, 0b101> hs;  // x halo: B_y, B_z;  y halo: B_x, B_z
+  PlaneStage<R, RR, TY, 3, 0b110, 0b101, TX> hs;  // x halo: B_y, B_z;  y halo: B_x, B_z
   hs.init(tx, ty, x0, y0, a.nx, a.ny);
 
   R q[3][Q];        // own column: planes it-RR .. it+RR
@@ -138,12 +138,12 @@ __device__ __forceinline__ void emf_tile(const FArgs<R>& a, int x0, int y0, int 
   }
 }
 
-template <typename R, int RR, int TY, int NBUF>
-__global__ __launch_bounds__(64 * TY) void k_emf(FArgs<R> a) {
-  __shared__ R sB[NBUF][3][TY + 2 * RR][64 + 2 * RR];
+template <typename R, int RR, int TY, int NBUF, int TX = 64>
+__global__ __launch_bounds__(TX * TY) void k_emf(FArgs<R> a) {
+  __shared__ R sB[NBUF][3][TY + 2 * RR][TX + 2 * RR];
   int x0, y0, z0;
-  tile_origin<TY>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
-  emf_tile<R, RR, TY, NBUF>(a, x0, y0, z0, sB);
+  tile_origin<TY, TX>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
+  emf_tile<R, RR, TY, NBUF, TX>(a, x0, y0, z0, sB);
 }
 
 // Persistent batched form: blocks stride over a tile list covering every member.
@@ -165,8 +165,8 @@ __global__ __launch_bounds__(64 * TY) void k_emf_batched(const FArgs<R>* __restr
 #ifndef IEQ_STAGE_WAVES
 #define IEQ_STAGE_WAVES 2
 #endif
-template <typename R, int RR, int TY, bool HALL, int NBUF = 2> struct StageLds {
-  static constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR;
+template <typename R, int RR, int TY, bool HALL, int NBUF = 2, int TX = 64> struct StageLds {
+  static constexpr int LX = TX + 2 * RR, LY = TY + 2 * RR;
   static constexpr int NE = HALL ? 3 : 1, LYE = HALL ? LY : 1, LXE = HALL ? LX : 1;
   R sB[NBUF][3][LY][LX];
   R sE[NBUF][NE][LYE][LXE];
@@ -180,8 +180,8 @@ template <typename R, int RR, int TY, bool HALL, int NBUF = 2> struct StageLds {
 // is loaded);  MODE 3: as 0 with P either Y or absent and no Q (no pointwise loads).  The
 // SSPRK stages are modes 2 and 3.  Loads follow the memory-counter discipline described above
 // emf_tile.
-template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ, int NBUF, int MODE>
-__device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, int z0, StageLds<R, RR, TY, HALL, NBUF>& L) {
+template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ, int NBUF, int MODE, int TX = 64>
+__device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, int z0, StageLds<R, RR, TY, HALL, NBUF, TX>& L) {
   constexpr int Q = 2 * RR + 1, PD = march_pd<R, RR>();
   constexpr int NC = HALL ? 6 : 3;                            // column loads: B, then E_x, E_y, E_z
   constexpr int NPW = (MODE == 0 ? 6 : MODE == 3 ? 0 : 3) + (ADJ ? 3 : 0);  // pointwise loads: P (b2), [Q], [G]
@@ -202,8 +202,8 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
   const R* Pp = MODE == 1 ? a.b2 : (a.P ? a.P : a.Y);
   const R* Qp = a.Q ? a.Q : a.Y;
   const R* Gp = a.G ? a.G : a.Y;
-  PlaneStage<R, RR, TY, 3> hsb;
-  PlaneStage<R, RR, TY, 3, 0b110, 0b101> hse;  // x halo: E_y, E_z;  y halo: E_x, E_z
+  PlaneStage<R, RR, TY, 3, 7, 7, TX> hsb;
+  PlaneStage<R, RR, TY, 3, 0b110, 0b101, TX> hse;  // x halo: E_y, E_z;  y halo: E_x, E_z
   hsb.init(tx, ty, x0, y0, a.nx, a.ny);
   if constexpr (HALL) hse.init(tx, ty, x0, y0, a.nx, a.ny);
 
@@ -443,13 +443,13 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
 }
 
 // FP64: 2 waves per SIMD (the stage state needs ~200 VGPRs); FP32 fits 4 waves without spills
-template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ, int NBUF, int MODE>
-__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : IEQ_STAGE_WAVES, 8))) void k_stage(
+template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ, int NBUF, int MODE, int TX = 64>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : IEQ_STAGE_WAVES, 8))) void k_stage(
     FArgs<R> a) {
-  __shared__ StageLds<R, RR, TY, HALL, NBUF> L;
+  __shared__ StageLds<R, RR, TY, HALL, NBUF, TX> L;
   int x0, y0, z0;
-  tile_origin<TY>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
-  stage_tile<R, RR, TY, HALL, HOD, CONV, ADJ, NBUF, MODE>(a, x0, y0, z0, L);
+  tile_origin<TY, TX>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
+  stage_tile<R, RR, TY, HALL, HOD, CONV, ADJ, NBUF, MODE, TX>(a, x0, y0, z0, L);
 }
 
 template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, int MODE>

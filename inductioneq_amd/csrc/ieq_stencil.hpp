@@ -52,10 +52,10 @@ __device__ inline int xcd_tile(int b, int nb) {
   return k * q + min(k, r) + b / kXcds;
 }
 
-// Linear tile index (x fastest, then y, then z-chunk) -> tile origin.
-template <int TY> __device__ inline void tile_origin(int t, int nx, int ny, int zc, int& x0, int& y0, int& z0) {
-  const int ntx = (nx + 63) / 64, nty = (ny + TY - 1) / TY;
-  x0 = (t % ntx) * 64;
+// Linear tile index (x fastest, then y, then z-chunk) -> origin of a TX x TY tile.
+template <int TY, int TX = 64> __device__ inline void tile_origin(int t, int nx, int ny, int zc, int& x0, int& y0, int& z0) {
+  const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
+  x0 = (t % ntx) * TX;
   t /= ntx;
   y0 = (t % nty) * TY;
   z0 = (t / nty) * zc;
@@ -122,13 +122,14 @@ template <typename R, int NC> __device__ __forceinline__ Comps<R, NC> comps(cons
   return r;
 }
 
-// Per-thread staging of one xy-plane of an NC-component field into an LDS tile with a
-// halo of RR cells.  The thread owns its column (x0+tx, y0+ty) and, at the tile edges,
+// Per-thread staging of one xy-plane of an NC-component field into an LDS tile (TX x TY
+// cells) with a halo of RR cells.  The thread owns its column (x0+tx, y0+ty) and, at the tile edges,
 // up to one x-halo cell and up to two y-halo cells (all coordinates wrap periodically).
 // XM / YM select the components whose x / y halo is staged (bit c = component c): a
 // component that is only differentiated along y needs no x halo, and vice versa.
-template <typename R, int RR, int TY, int NC, int XM = (1 << NC) - 1, int YM = (1 << NC) - 1> struct PlaneStage {
-  static constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR;
+template <typename R, int RR, int TY, int NC, int XM = (1 << NC) - 1, int YM = (1 << NC) - 1, int TX = 64>
+struct PlaneStage {
+  static constexpr int LX = TX + 2 * RR, LY = TY + 2 * RR;
   // With TY >= 2*RR a thread stages at most one y-halo cell (top rows or bottom rows).
   static constexpr bool ONE_Y = TY >= 2 * RR;
   bool hx, ht, hb;
@@ -138,7 +139,7 @@ template <typename R, int RR, int TY, int NC, int XM = (1 << NC) - 1, int YM = (
 
   __device__ void init(int tx, int ty, int x0, int y0, int nx, int ny) {
     const int xw = wrap(x0 + tx, nx), yw = wrap(y0 + ty, ny);
-    hx = tx < RR || tx >= 64 - RR;
+    hx = tx < RR || tx >= TX - RR;
     ht = ty < RR;
     hb = ty >= TY - RR;
     lcx = tx < RR ? tx : tx + 2 * RR;
@@ -220,7 +221,7 @@ inline int stage_mode(bool two_n, int P, int Q, int Y) {
 // Fused plan of one step (same arithmetic as the reference schedule, time kernels dropped).
 bool fused_plan(const std::string& I, int hall, std::vector<StageSpec>& v, int& nf);
 
-// Fused forward launches (TY in {4, 8}, RR in {1, 2, 3}).
+// Fused forward launches (RR in {1, 2, 3}; tile shape code 4: 64 x 4, 8: 64 x 8, 32: 32 x 8).
 template <typename R> void launch_emf(int rr, int ty, const FArgs<R>& a, hipStream_t s);
 template <typename R> void launch_stage(int rr, int ty, const FArgs<R>& a, bool hall, bool hod, hipStream_t s);
 // z-chunk that makes the stage kernel's tile grid one round of resident workgroups.
