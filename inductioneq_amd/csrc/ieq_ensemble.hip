@@ -115,6 +115,11 @@ template <typename R> std::vector<FArgs<R>> ensemble_args(Ensemble& e) {
   return out;
 }
 
+// Tile shapes by code, as for the single-grid launches: 4 -> 64 x 4, 8 -> 64 x 8, 32 -> 32 x 8
+// (32-wide tiles waste fewer lanes on members whose nx is not a multiple of 64).
+constexpr int ens_tx(int code) { return code == 32 ? 32 : 64; }
+constexpr int ens_ty(int code) { return code == 32 ? 8 : code; }
+
 std::vector<int4> ensemble_tiles(const Ensemble& e, int ty, int zc, int order) {
   std::vector<int> idx(e.m.size());
   for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
@@ -126,47 +131,52 @@ std::vector<int4> ensemble_tiles(const Ensemble& e, int ty, int zc, int order) {
   for (int i : idx) {
     const ieq_member& mb = e.m[i];
     for (int z = 0; z < mb.nz; z += zc)
-      for (int y = 0; y < mb.ny; y += ty)
-        for (int x = 0; x < mb.nx; x += 64) t.push_back(make_int4(x, y, z, i));
+      for (int y = 0; y < mb.ny; y += ens_ty(ty))
+        for (int x = 0; x < mb.nx; x += ens_tx(ty)) t.push_back(make_int4(x, y, z, i));
   }
   return t;
 }
 
-template <typename R, int RR, int TY, int MODE> const void* ens_stage_kernel(const Ensemble& e) {
+template <typename R, int RR, int TY, int MODE, int TX> const void* ens_stage_kernel(const Ensemble& e) {
   const bool h = e.hall, d = e.hod;
   if (e.conv) {
-    if (h && d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, true, true, MODE>);
-    if (h) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, false, true, MODE>);
-    if (d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, true, true, MODE>);
-    return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, false, true, MODE>);
+    if (h && d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, true, true, MODE, TX>);
+    if (h) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, false, true, MODE, TX>);
+    if (d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, true, true, MODE, TX>);
+    return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, false, true, MODE, TX>);
   }
-  if (h && d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, true, false, MODE>);
-  if (h) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, false, false, MODE>);
-  if (d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, true, false, MODE>);
-  return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, false, false, MODE>);
+  if (h && d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, true, false, MODE, TX>);
+  if (h) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, true, false, false, MODE, TX>);
+  if (d) return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, true, false, MODE, TX>);
+  return reinterpret_cast<const void*>(&k_stage_batched<R, RR, TY, false, false, false, MODE, TX>);
 }
 
 // The EMF kernel or the stage kernel of a plan entry (stage_mode on its register indices).
-template <typename R, int RR, int TY> const void* ens_kernel_t(const Ensemble& e, const StageSpec& st) {
-  if (st.kind == 3) return reinterpret_cast<const void*>(&k_emf_batched<R, RR, TY>);
+template <typename R, int RR, int TY, int TX> const void* ens_kernel_t(const Ensemble& e, const StageSpec& st) {
+  if (st.kind == 3) return reinterpret_cast<const void*>(&k_emf_batched<R, RR, TY, TX>);
   switch (stage_mode(st.kind == 4, st.P, st.Q, st.Y)) {
-    case 1: return ens_stage_kernel<R, RR, TY, 1>(e);
-    case 2: return ens_stage_kernel<R, RR, TY, 2>(e);
-    case 3: return ens_stage_kernel<R, RR, TY, 3>(e);
-    default: return ens_stage_kernel<R, RR, TY, 0>(e);
+    case 1: return ens_stage_kernel<R, RR, TY, 1, TX>(e);
+    case 2: return ens_stage_kernel<R, RR, TY, 2, TX>(e);
+    case 3: return ens_stage_kernel<R, RR, TY, 3, TX>(e);
+    default: return ens_stage_kernel<R, RR, TY, 0, TX>(e);
   }
 }
 
 // Kernel of a plan entry (kind 3: EMF, otherwise the stage kernel) for the ensemble's settings.
-template <typename R> const void* ens_kernel(const Ensemble& e, int ty, const StageSpec& st) {
-  if (ty == 4) {
-    if (e.rr == 1) return ens_kernel_t<R, 1, 4>(e, st);
-    if (e.rr == 2) return ens_kernel_t<R, 2, 4>(e, st);
-    return ens_kernel_t<R, 3, 4>(e, st);
+template <typename R> const void* ens_kernel(const Ensemble& e, int code, const StageSpec& st) {
+  if (code == 32) {
+    if (e.rr == 1) return ens_kernel_t<R, 1, 8, 32>(e, st);
+    if (e.rr == 2) return ens_kernel_t<R, 2, 8, 32>(e, st);
+    return ens_kernel_t<R, 3, 8, 32>(e, st);
   }
-  if (e.rr == 1) return ens_kernel_t<R, 1, 8>(e, st);
-  if (e.rr == 2) return ens_kernel_t<R, 2, 8>(e, st);
-  return ens_kernel_t<R, 3, 8>(e, st);
+  if (code == 4) {
+    if (e.rr == 1) return ens_kernel_t<R, 1, 4, 64>(e, st);
+    if (e.rr == 2) return ens_kernel_t<R, 2, 4, 64>(e, st);
+    return ens_kernel_t<R, 3, 4, 64>(e, st);
+  }
+  if (e.rr == 1) return ens_kernel_t<R, 1, 8, 64>(e, st);
+  if (e.rr == 2) return ens_kernel_t<R, 2, 8, 64>(e, st);
+  return ens_kernel_t<R, 3, 8, 64>(e, st);
 }
 
 template <typename R> hipError_t ens_launch_steps(Ensemble& e, hipStream_t s) {
@@ -179,7 +189,7 @@ template <typename R> hipError_t ens_launch_steps(Ensemble& e, hipStream_t s) {
       ++k;
       void* kargs[3] = {(void*)&a, (void*)&e.dtiles, (void*)&e.ntiles};
       const int grid = st.kind == 3 ? e.grid_emf : e.grid_stage;
-      hipError_t r = hipLaunchKernel(ens_kernel<R>(e, e.ty, st), dim3(grid), dim3(64, e.ty), kargs, 0, s);
+      hipError_t r = hipLaunchKernel(ens_kernel<R>(e, e.ty, st), dim3(grid), dim3(ens_tx(e.ty), ens_ty(e.ty)), kargs, 0, s);
       if (r != hipSuccess) return r;
     }
   return hipSuccess;
@@ -209,12 +219,13 @@ template <typename R> int ens_configure(Ensemble& e, int ty, int zc, int order, 
   // (the smallest occupancy over the plan's stage kernels)
   int occ_emf = 1, occ_stage = 1 << 30;
   const StageSpec emf_spec{3, -1, -1, -1, 0, 0, 0, 0};
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_emf, ens_kernel<R>(e, ty, emf_spec), 64 * ty, 0) != hipSuccess)
+  const int nthreads = ens_tx(ty) * ens_ty(ty);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_emf, ens_kernel<R>(e, ty, emf_spec), nthreads, 0) != hipSuccess)
     return err = "occupancy query failed", -4;
   for (const StageSpec& st : e.plan) {
     if (st.kind == 3) continue;
     int o = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, ens_kernel<R>(e, ty, st), 64 * ty, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, ens_kernel<R>(e, ty, st), nthreads, 0) != hipSuccess)
       return err = "occupancy query failed", -4;
     occ_stage = std::min(occ_stage, o);
   }
@@ -262,7 +273,7 @@ template <typename R> int ens_tune(Ensemble& e, std::string& err) {
   hipEvent_t t0, t1;
   (void)hipEventCreate(&t0);
   (void)hipEventCreate(&t1);
-  const int tys[2] = {4, 8}, zcs[3] = {4, 8, 16};
+  const int tys[3] = {4, 8, 32}, zcs[3] = {4, 8, 16};
   for (int ty : tys)
     for (int zc : zcs)
       for (int ord = 0; ord < 2; ++ord) {

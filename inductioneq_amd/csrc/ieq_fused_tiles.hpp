@@ -147,15 +147,15 @@ __global__ __launch_bounds__(TX * TY) void k_emf(FArgs<R> a) {
 }
 
 // Persistent batched form: blocks stride over a tile list covering every member.
-template <typename R, int RR, int TY>
-__global__ __launch_bounds__(64 * TY) void k_emf_batched(const FArgs<R>* __restrict__ args, const int4* __restrict__ tiles,
+template <typename R, int RR, int TY, int TX = 64>
+__global__ __launch_bounds__(TX * TY) void k_emf_batched(const FArgs<R>* __restrict__ args, const int4* __restrict__ tiles,
                                                          int ntiles) {
-  __shared__ R sB[2][3][TY + 2 * RR][64 + 2 * RR];
+  __shared__ R sB[2][3][TY + 2 * RR][TX + 2 * RR];
   const int chunk = (ntiles + kXcds - 1) / kXcds, slots = gridDim.x / kXcds;  // gridDim.x is a multiple of 8
   const int first = (blockIdx.x % kXcds) * chunk, last = min(first + chunk, ntiles);
   for (int t = first + (int)blockIdx.x / kXcds; t < last; t += slots) {
     const int4 d = tiles[t];
-    emf_tile<R, RR, TY>(args[d.w], d.x, d.y, d.z, sB);
+    emf_tile<R, RR, TY, 2, TX>(args[d.w], d.x, d.y, d.z, sB);
     lds_barrier();
   }
 }
@@ -452,15 +452,15 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(
   stage_tile<R, RR, TY, HALL, HOD, CONV, ADJ, NBUF, MODE, TX>(a, x0, y0, z0, L);
 }
 
-template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, int MODE>
-__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : IEQ_STAGE_WAVES, 8))) void k_stage_batched(
+template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, int MODE, int TX = 64>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : IEQ_STAGE_WAVES, 8))) void k_stage_batched(
     const FArgs<R>* __restrict__ args, const int4* __restrict__ tiles, int ntiles) {
-  __shared__ StageLds<R, RR, TY, HALL> L;
+  __shared__ StageLds<R, RR, TY, HALL, 2, TX> L;
   const int chunk = (ntiles + kXcds - 1) / kXcds, slots = gridDim.x / kXcds;  // gridDim.x is a multiple of 8
   const int first = (blockIdx.x % kXcds) * chunk, last = min(first + chunk, ntiles);
   for (int t = first + (int)blockIdx.x / kXcds; t < last; t += slots) {
     const int4 d = tiles[t];
-    stage_tile<R, RR, TY, HALL, HOD, CONV, false, 2, MODE>(args[d.w], d.x, d.y, d.z, L);
+    stage_tile<R, RR, TY, HALL, HOD, CONV, false, 2, MODE, TX>(args[d.w], d.x, d.y, d.z, L);
     lds_barrier();
   }
 }
