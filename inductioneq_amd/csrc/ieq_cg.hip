@@ -83,6 +83,18 @@ struct LapArgs {
   unsigned int nblocks;
 };
 
+// Minimum waves per SIMD the register allocation of a 16-row Laplacian tile must allow.  A
+// workgroup of 64 x TY threads puts TY/4 waves on each SIMD, so it needs at least 4 (the
+// unconstrained FP64 kernel at half-width 4 takes 133 VGPRs, 3 waves per SIMD, and would not
+// launch).  With one plane of loads in flight the kernels up to half-width 5 fit without
+// spilling (half-width 4: FP64 120 VGPRs, 4 waves; FP32 74 VGPRs, 6 waves).
+#ifndef IEQ_LAP_WAVES64
+#define IEQ_LAP_WAVES64 4
+#endif
+#ifndef IEQ_LAP_WAVES32
+#define IEQ_LAP_WAVES32 6
+#endif
+
 // Fused direction update and Laplacian on [off, n-off)^3:
 //   u' = beta u + r (beta = rr/old),  q = -Lap u',  partial u'.q
 // u' is written to a second buffer (neighbouring blocks still read u), so the direction
@@ -97,7 +109,7 @@ struct LapArgs {
 // Tiles are handed out per XCD in contiguous ranges (x fastest, then y, then z-chunk), so
 // the halo rows of y-neighbours are read through one L2.
 template <typename R, int HW, int TY, int PD>
-__global__ __launch_bounds__(64 * TY) void k_cg_lap(LapArgs a, const R* __restrict__ u, const R* __restrict__ r,
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY >= 16 ? (sizeof(R) == 4 ? IEQ_LAP_WAVES32 : IEQ_LAP_WAVES64) : 1, 8))) void k_cg_lap(LapArgs a, const R* __restrict__ u, const R* __restrict__ r,
                                                     R* __restrict__ un, R* __restrict__ q, CgState* st,
                                                     const double* rr_part, unsigned int n_rr, double* part, int iter) {
   constexpr int LX = 64 + 2 * HW, LY = TY + 2 * HW, Q = 2 * HW + 1;
@@ -345,13 +357,15 @@ __global__ __launch_bounds__(256) void k_cg_init_state(const double* part, unsig
 }
 
 #ifndef IEQ_LAP_PD
-#define IEQ_LAP_PD 2
+#define IEQ_LAP_PD 1
 #endif
 constexpr int kLapPD = IEQ_LAP_PD;  // planes of the Laplacian's z-march in flight per thread
 #ifndef IEQ_LAP_TY
-#define IEQ_LAP_TY 8
+#define IEQ_LAP_TY 16
 #endif
-constexpr int kLapTY = IEQ_LAP_TY;  // rows of a Laplacian tile (64 x kLapTY threads)
+// Rows of a Laplacian tile (64 x TY threads) for stencil half-width hw: 16 up to hw = 5; the
+// wider stencils hold more registers (they would spill at 4 waves per SIMD) and run 8 rows.
+constexpr int lap_ty(int hw) { return hw <= 5 ? IEQ_LAP_TY : 8; }
 
 template <typename R, int HW, int TY>
 void launch_lap(const LapArgs& a, const R* u, const R* r, R* un, R* q, CgState* st, const double* rr_part, unsigned int n_rr,
@@ -398,17 +412,18 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
   const int hw = (wl - 1) / 2;
   if (hw < 1 || hw > 7 || info.periodic || 2 * nbl + 1 > kLapRows || !offsets_fit(p.nx, p.ny, p.nz, sizeof(R)))
     return 1;  // not covered by the fused loop
-  const int TY = kLapTY;
+  const int TY = lap_ty(hw);
   const int gx = p.nx - 2 * p.bnodes, gy = p.ny - 2 * p.bnodes, gz = p.nz - 2 * p.bnodes;
   int zc = 8;
-  switch (hw) {
-    case 1: zc = lap_zc<R, 1, kLapTY>(gx, gy, gz); break;
-    case 2: zc = lap_zc<R, 2, kLapTY>(gx, gy, gz); break;
-    case 3: zc = lap_zc<R, 3, kLapTY>(gx, gy, gz); break;
-    case 4: zc = lap_zc<R, 4, kLapTY>(gx, gy, gz); break;
-    case 5: zc = lap_zc<R, 5, kLapTY>(gx, gy, gz); break;
-    case 6: zc = lap_zc<R, 6, kLapTY>(gx, gy, gz); break;
-    default: zc = lap_zc<R, 7, kLapTY>(gx, gy, gz); break;
+  if (const char* e = getenv("IEQ_LAP_ZC")) zc = std::max(1, atoi(e));
+  else switch (hw) {
+    case 1: zc = lap_zc<R, 1, lap_ty(1)>(gx, gy, gz); break;
+    case 2: zc = lap_zc<R, 2, lap_ty(2)>(gx, gy, gz); break;
+    case 3: zc = lap_zc<R, 3, lap_ty(3)>(gx, gy, gz); break;
+    case 4: zc = lap_zc<R, 4, lap_ty(4)>(gx, gy, gz); break;
+    case 5: zc = lap_zc<R, 5, lap_ty(5)>(gx, gy, gz); break;
+    case 6: zc = lap_zc<R, 6, lap_ty(6)>(gx, gy, gz); break;
+    default: zc = lap_zc<R, 7, lap_ty(7)>(gx, gy, gz); break;
   }
   const unsigned int lap_blocks = (unsigned)(((gx + 63) / 64) * ((gy + TY - 1) / TY) * ((gz + zc - 1) / zc));
   const unsigned int vec_blocks = 1024;
@@ -471,13 +486,13 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
       const R* u = ub[it & 1];
       R* un = ub[(it + 1) & 1];
       switch (hw) {
-        case 1: launch_lap<R, 1, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        case 2: launch_lap<R, 2, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        case 3: launch_lap<R, 3, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        case 4: launch_lap<R, 4, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        case 5: launch_lap<R, 5, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        case 6: launch_lap<R, 6, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
-        default: launch_lap<R, 7, kLapTY>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 1: launch_lap<R, 1, lap_ty(1)>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 2: launch_lap<R, 2, lap_ty(2)>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 3: launch_lap<R, 3, lap_ty(3)>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 4: launch_lap<R, 4, lap_ty(4)>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 5: launch_lap<R, 5, lap_ty(5)>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        case 6: launch_lap<R, 6, lap_ty(6)>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
+        default: launch_lap<R, 7, lap_ty(7)>(la, u, r, un, q, st, part_rr, vec_blocks, part_uq, it, s); break;
       }
       k_cg_update<R><<<vec_blocks, kBlk, 0, s>>>(phi, un, r, q, n, st, part_uq, lap_blocks, part_rr, it);
     }
