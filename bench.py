@@ -552,7 +552,7 @@ def bench_cg(a):
         "fp32_vs_fp64_cleaned_field_max_rel_diff": diff,
         "roofline": {"bound": "hbm", "achieved": round(alg / it_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(alg / it_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "one CG iteration (k_cg_lap + k_cg_update + two one-workgroup reductions), 40 B/cell algorithmic FP32"},
+                     "kernel": "one CG iteration (k_cg_lap + k_cg_update, FP64 dot partials summed inside both), 40 B/cell algorithmic FP32"},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
