@@ -27,17 +27,20 @@ def _setup(ops, lap, n, real):
     return Program(cfg), Oracle(ocfg), cfg, aos
 
 
-@pytest.mark.parametrize("ops,lap,n", [
-    ("4thOrder", "WIDE_STENCIL_DIRICHLET", (40, 36, 30)),       # fused loop (half-width 4)
-    ("2ndOrder", "NARROW_STENCIL_DIRICHLET", (33, 31, 35)),     # fused loop (half-width 1)
-    ("2ndOrder", "WIDE_STENCIL_LNS", (24, 26, 22)),             # fused loop, BNODES = 0
-    ("6thOrderExtended", "WIDE_STENCIL_DIRICHLET", (30, 28, 26)),  # half-width 9: reference kernel sequence
+@pytest.mark.parametrize("ops,lap,n,maxit", [
+    ("4thOrder", "WIDE_STENCIL_DIRICHLET", (40, 36, 30), 60),       # fused loop (half-width 4, 16-row tiles)
+    ("2ndOrder", "NARROW_STENCIL_DIRICHLET", (33, 31, 35), 60),     # fused loop (half-width 1)
+    ("2ndOrder", "WIDE_STENCIL_LNS", (24, 26, 22), 60),             # fused loop, BNODES = 0
+    ("2ndOrderExtended", "WIDE_STENCIL_LNS", (26, 24, 28), 60),     # fused loop (half-width 3)
+    ("4thOrder", "WIDE_STENCIL_LNS", (30, 32, 28), 12),             # fused loop (half-width 5); not M-symmetric, few iterations
+    ("4thOrderExtended", "WIDE_STENCIL_DIRICHLET", (34, 30, 32), 60),  # fused loop (half-width 6, 8-row tiles)
+    ("6thOrderExtended", "WIDE_STENCIL_DIRICHLET", (30, 28, 26), 60),  # half-width 9: reference kernel sequence
 ])
-def test_cg_matches_oracle_fp64(ops, lap, n):
+def test_cg_matches_oracle_fp64(ops, lap, n, maxit):
     import torch
 
     prog, orc, cfg, aos = _setup(ops, lap, n, "double")
-    thr, maxit = 1e-6, 60
+    thr = 1e-6
     hb = aos.copy()
     want = clean_div_cg(orc.cfg, orc, hb, thr, maxit)
     db = prog.upload_real4(aos)
