@@ -178,10 +178,10 @@ int ieq_ensemble_create(ieq_ensemble **ens, ieq_program *prog, const ieq_member 
  * 2N schemes, which must get an even nsteps).  b1 of every member holds the state on return. */
 int ieq_ensemble_step(ieq_ensemble *ens, long long nsteps, double *kernel_seconds);
 
-/* Chosen launch configuration: tile shape code (4: 64 x 4 cells, 8: 64 x 8, 32: 32 x 8), z-chunk, order (0 member order, 1 largest first;
- * +2 when the step is replayed from a hipGraph rather than launched directly, whichever
- * timed faster). */
-int ieq_ensemble_config(const ieq_ensemble *ens, int *tile_rows, int *z_chunk, int *order);
+/* Chosen launch configuration: tile shape code (4: 64 x 4 cells, 8: 64 x 8, 32: 32 x 8),
+ * z-chunk, order (0 member order, 1 largest first; +2 when the step is replayed from a
+ * hipGraph rather than launched directly, whichever timed faster). */
+int ieq_ensemble_config(const ieq_ensemble *ens, int *tile_shape, int *z_chunk, int *order);
 
 void ieq_ensemble_release(ieq_ensemble *ens);
 

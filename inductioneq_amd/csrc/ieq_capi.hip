@@ -770,9 +770,9 @@ int ieq_ensemble_step(ieq_ensemble* ens, long long nsteps, double* kernel_second
   return ieq::ensemble_step(ens->e, nsteps, kernel_seconds, ens->prog->err);
 }
 
-int ieq_ensemble_config(const ieq_ensemble* ens, int* tile_rows, int* z_chunk, int* order) {
-  if (!ens || !tile_rows || !z_chunk || !order) return IEQ_ERR_ARGS;
-  ieq::ensemble_config(ens->e, tile_rows, z_chunk, order);
+int ieq_ensemble_config(const ieq_ensemble* ens, int* tile_shape, int* z_chunk, int* order) {
+  if (!ens || !tile_shape || !z_chunk || !order) return IEQ_ERR_ARGS;
+  ieq::ensemble_config(ens->e, tile_shape, z_chunk, order);
   return IEQ_OK;
 }
 
