@@ -27,6 +27,49 @@ EXPORTS = [
     "ieq_adjoint_release",
 ]
 
+# typed per-kernel entry points declared in include/inductioneq_kernels.h, by buffer count
+# (ieq_<name>(prog, g, l, buffers..., kernel_seconds))
+KERNEL_BUFFERS = {}
+for _n in (
+    "init_b", "init_u", "init_rho",
+):
+    KERNEL_BUFFERS[_n] = 1
+for _n in (
+    "calc_div_divcleaning", "calc_laplace_divcleaning", "projector_divcleaning", "analytical_b", "norm2",
+    "norm2_S", "calc_div", "calc_curl", "calc_grad",
+):
+    KERNEL_BUFFERS[_n] = 2
+for _n in (
+    "calc_axpy", "calc_xpay", "dot_product", "norm2_diff",
+):
+    KERNEL_BUFFERS[_n] = 3
+for _n in (
+    "calc_time_2_args", "calc_u_2_args", "calc_curlB_rho_1_2args", "calc_curlB_rho_2_2args",
+    "CarpenterKennedy2N54_1a", "CarpenterKennedy2N54_1b", "CarpenterKennedy2N54_2a", "CarpenterKennedy2N54_2b",
+    "CarpenterKennedy2N54_3a", "CarpenterKennedy2N54_3b", "CarpenterKennedy2N54_4a", "CarpenterKennedy2N54_4b",
+    "CarpenterKennedy2N54_5a", "CarpenterKennedy2N54_5b", "ToulorgeDesmet2N84F_1a", "ToulorgeDesmet2N84F_1b",
+    "ToulorgeDesmet2N84F_2a", "ToulorgeDesmet2N84F_2b", "ToulorgeDesmet2N84F_3a", "ToulorgeDesmet2N84F_3b",
+    "ToulorgeDesmet2N84F_4a", "ToulorgeDesmet2N84F_4b", "ToulorgeDesmet2N84F_5a", "ToulorgeDesmet2N84F_5b",
+    "ToulorgeDesmet2N84F_6a", "ToulorgeDesmet2N84F_6b", "ToulorgeDesmet2N84F_7a", "ToulorgeDesmet2N84F_7b",
+    "ToulorgeDesmet2N84F_8a", "ToulorgeDesmet2N84F_8b",
+):
+    KERNEL_BUFFERS[_n] = 6
+for _n in (
+    "calc_time_3_args", "calc_u_3_args", "calc_curlB_rho_1_3args", "calc_curlB_rho_2_3args",
+    "calc_curlB_rho_3_3args", "SSPRK33_1", "SSPRK33_2a", "SSPRK33_2b", "SSPRK33_3a", "SSPRK33_3b", "SSPRK104_01",
+    "SSPRK104_02a", "SSPRK104_02b", "SSPRK104_03a", "SSPRK104_03b", "SSPRK104_04a", "SSPRK104_04b", "SSPRK104_05a",
+    "SSPRK104_05b", "SSPRK104_06", "SSPRK104_07", "SSPRK104_08a", "SSPRK104_08b", "SSPRK104_09a", "SSPRK104_09b",
+    "SSPRK104_10a", "SSPRK104_10b", "SSPRK104_11a", "SSPRK104_11b", "KennedyCarpenterLewis2R54C_1a",
+    "KennedyCarpenterLewis2R54C_1b", "KennedyCarpenterLewis2R54C_2a", "KennedyCarpenterLewis2R54C_2b",
+    "KennedyCarpenterLewis2R54C_3a", "KennedyCarpenterLewis2R54C_3b", "KennedyCarpenterLewis2R54C_4a",
+    "KennedyCarpenterLewis2R54C_4b", "KennedyCarpenterLewis2R54C_5", "CalvoFrancoRandez2R64_1a",
+    "CalvoFrancoRandez2R64_1b", "CalvoFrancoRandez2R64_2a", "CalvoFrancoRandez2R64_2b", "CalvoFrancoRandez2R64_3a",
+    "CalvoFrancoRandez2R64_3b", "CalvoFrancoRandez2R64_4a", "CalvoFrancoRandez2R64_4b", "CalvoFrancoRandez2R64_5a",
+    "CalvoFrancoRandez2R64_5b", "CalvoFrancoRandez2R64_6",
+):
+    KERNEL_BUFFERS[_n] = 7
+EXPORTS += ["ieq_" + _n for _n in KERNEL_BUFFERS]
+
 _NOT_INT = ("ieq_program_release", "ieq_last_error", "ieq_vector_field_bytes", "ieq_scalar_field_bytes",
             "ieq_member_stride", "ieq_ensemble_release", "ieq_adjoint_release")
 
@@ -111,6 +154,8 @@ def load() -> ctypes.CDLL:
     lib.ieq_adjoint_counts.argtypes = [P, llp, llp]
     lib.ieq_adjoint_release.argtypes = [P]
     lib.ieq_adjoint_release.restype = None
+    for name, nbuf in KERNEL_BUFFERS.items():
+        getattr(lib, "ieq_" + name).argtypes = [P, u32x3, u32x3] + [P] * nbuf + [ctypes.POINTER(ctypes.c_double)]
     for fn in EXPORTS:
         if fn not in _NOT_INT:
             getattr(lib, fn).restype = ctypes.c_int

@@ -165,6 +165,9 @@ __global__ __launch_bounds__(TX * TY) void k_emf_batched(const FArgs<R>* __restr
 #ifndef IEQ_STAGE_WAVES
 #define IEQ_STAGE_WAVES 2
 #endif
+#ifndef IEQ_STAGE_WAVES32
+#define IEQ_STAGE_WAVES32 4
+#endif
 template <typename R, int RR, int TY, bool HALL, int NBUF = 2, int TX = 64> struct StageLds {
   static constexpr int LX = TX + 2 * RR, LY = TY + 2 * RR;
   static constexpr int NE = HALL ? 3 : 1, LYE = HALL ? LY : 1, LXE = HALL ? LX : 1;
@@ -444,7 +447,7 @@ __device__ __forceinline__ void stage_tile(const FArgs<R>& a, int x0, int y0, in
 
 // FP64: 2 waves per SIMD (the stage state needs ~200 VGPRs); FP32 fits 4 waves without spills
 template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, bool ADJ, int NBUF, int MODE, int TX = 64>
-__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : IEQ_STAGE_WAVES, 8))) void k_stage(
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? IEQ_STAGE_WAVES32 : IEQ_STAGE_WAVES, 8))) void k_stage(
     FArgs<R> a) {
   __shared__ StageLds<R, RR, TY, HALL, NBUF, TX> L;
   int x0, y0, z0;
@@ -453,7 +456,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(
 }
 
 template <typename R, int RR, int TY, bool HALL, bool HOD, bool CONV, int MODE, int TX = 64>
-__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : IEQ_STAGE_WAVES, 8))) void k_stage_batched(
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? IEQ_STAGE_WAVES32 : IEQ_STAGE_WAVES, 8))) void k_stage_batched(
     const FArgs<R>* __restrict__ args, const int4* __restrict__ tiles, int ntiles) {
   __shared__ StageLds<R, RR, TY, HALL, 2, TX> L;
   const int chunk = (ntiles + kXcds - 1) / kXcds, slots = gridDim.x / kXcds;  // gridDim.x is a multiple of 8

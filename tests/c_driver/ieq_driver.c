@@ -5,13 +5,15 @@
  *
  * Builds the program from the generate_settings string and module list, uploads the
  * AoS REAL4 field of in.bin (num_nodes_pad x 4 REALs), runs nsteps steps of the
- * integrator once through ieq_step and once through ieq_run with the reference kernel
- * list, and writes both results (AoS REAL4, in that order) to out.bin. */
+ * integrator three times: through ieq_step, through ieq_run with the reference kernel
+ * list, and through the typed per-kernel entry points (inductioneq_kernels.h) with the
+ * same list; writes the three results (AoS REAL4, in that order) to out.bin. */
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "../../include/inductioneq.h"
+#include "../../include/inductioneq_kernels.h"
 
 static int fail(ieq_program *p, const char *what) {
   fprintf(stderr, "ieq_driver: %s: %s\n", what, ieq_last_error(p));
@@ -26,6 +28,16 @@ static const char *ck54[] = {
     "calc_curlB_rho_1_2args", "CarpenterKennedy2N54_3a", "CarpenterKennedy2N54_3b",
     "calc_curlB_rho_1_2args", "CarpenterKennedy2N54_4a", "CarpenterKennedy2N54_4b",
     "calc_curlB_rho_1_2args", "CarpenterKennedy2N54_5a", "CarpenterKennedy2N54_5b", "calc_time_2_args"};
+
+/* The same list as typed calls (b1, b2, curlB_rho, u, rho, time). */
+typedef int (*ieq_kernel6)(ieq_program *, const uint32_t *, const uint32_t *, void *, void *, void *, void *, void *,
+                           void *, double *);
+static const ieq_kernel6 ck54_typed[] = {
+    ieq_calc_curlB_rho_1_2args, ieq_CarpenterKennedy2N54_1a, ieq_CarpenterKennedy2N54_1b,
+    ieq_calc_curlB_rho_1_2args, ieq_CarpenterKennedy2N54_2a, ieq_CarpenterKennedy2N54_2b,
+    ieq_calc_curlB_rho_1_2args, ieq_CarpenterKennedy2N54_3a, ieq_CarpenterKennedy2N54_3b,
+    ieq_calc_curlB_rho_1_2args, ieq_CarpenterKennedy2N54_4a, ieq_CarpenterKennedy2N54_4b,
+    ieq_calc_curlB_rho_1_2args, ieq_CarpenterKennedy2N54_5a, ieq_CarpenterKennedy2N54_5b, ieq_calc_time_2_args};
 
 int main(int argc, char **argv) {
   int sep = 0;
@@ -49,14 +61,14 @@ int main(int argc, char **argv) {
   ieq_program_info info;
   if (ieq_program_get_info(p, &info) != IEQ_OK) return fail(p, "info");
   const size_t n = (size_t)info.num_nodes_pad, rb = (size_t)info.real_bytes;
-  void *host = malloc(n * 4 * rb), *out = malloc(2 * n * 4 * rb);
+  void *host = malloc(n * 4 * rb), *out = malloc(3 * n * 4 * rb);
   FILE *f = fopen(argv[sep + 3], "rb");
   if (!f || fread(host, rb, n * 4, f) != n * 4) {
     fprintf(stderr, "ieq_driver: cannot read %s\n", argv[sep + 3]);
     return 1;
   }
   fclose(f);
-  for (int path = 0; path < 2; ++path) {
+  for (int path = 0; path < 3; ++path) {
     /* b1, b2, curlB_rho, u, rho, time (kernel_time_integrator.cl:1023-1026) */
     void *b1, *b2, *J, *u, *rho, *t;
     const size_t vb = ieq_vector_field_bytes(p), sb = ieq_scalar_field_bytes(p);
@@ -77,15 +89,21 @@ int main(int argc, char **argv) {
       if (ieq_step(p, integrator, nsteps, 0, args, 6, NULL) != IEQ_OK) return fail(p, "ieq_step");
     } else {
       const uint32_t g[3] = {(uint32_t)info.nx, (uint32_t)info.ny, (uint32_t)info.nz}, l[3] = {(uint32_t)info.w_size, 1, 1};
-      for (long long s = 0; s < nsteps; ++s)
-        if (ieq_run(p, ck54, (int)(sizeof(ck54) / sizeof(ck54[0])), g, l, args, 6, NULL) != IEQ_OK) return fail(p, "ieq_run");
+      for (long long s = 0; s < nsteps; ++s) {
+        if (path == 1) {
+          if (ieq_run(p, ck54, (int)(sizeof(ck54) / sizeof(ck54[0])), g, l, args, 6, NULL) != IEQ_OK) return fail(p, "ieq_run");
+        } else {
+          for (size_t k = 0; k < sizeof(ck54_typed) / sizeof(ck54_typed[0]); ++k)
+            if (ck54_typed[k](p, g, l, b1, b2, J, u, rho, t, NULL) != IEQ_OK) return fail(p, ck54[k]);
+        }
+      }
     }
     if (ieq_download_real4(p, (char *)out + path * n * 4 * rb, b1, n) != IEQ_OK) return fail(p, "download");
     void *all[6] = {b1, b2, J, u, rho, t};
     for (int k = 0; k < 6; ++k) ieq_free(p, all[k]);
   }
   f = fopen(argv[sep + 4], "wb");
-  if (!f || fwrite(out, rb, 2 * n * 4, f) != 2 * n * 4) {
+  if (!f || fwrite(out, rb, 3 * n * 4, f) != 3 * n * 4) {
     fprintf(stderr, "ieq_driver: cannot write %s\n", argv[sep + 4]);
     return 1;
   }

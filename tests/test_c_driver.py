@@ -1,8 +1,9 @@
 """The C ABI from a plain C program (tests/c_driver/ieq_driver.c): no Python, no torch.
 
 CPU: the driver compiles and links against the in-tree library.  GPU: it runs two
-CarpenterKennedy2N54 steps of the periodic Hall problem through ieq_step and through
-ieq_run with the reference kernel list; both results match the oracle to 1e-12.
+CarpenterKennedy2N54 steps of the periodic Hall problem through ieq_step, through ieq_run
+with the reference kernel list and through the typed per-kernel entry points with the same
+list; all three results match the oracle to 1e-12.
 """
 import math
 import os
@@ -51,12 +52,12 @@ def test_c_driver_matches_oracle(tmp_path):
                                                                          str(inp), str(out)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    got = np.fromfile(out, dtype=np.float64).reshape(2, cfg.num_nodes_pad, 4)
+    got = np.fromfile(out, dtype=np.float64).reshape(3, cfg.num_nodes_pad, 4)
     orc = Oracle(cfg)
     names, nf = rk_schedule("CarpenterKennedy2N54", True, False)
     hb = [host.copy(), orc.vec()]
     orc.run(names * nsteps, hb + [orc.vec(), orc.vec(), orc.scal(1.0), orc.time()])
     want = hb[0][: cfg.num_nodes, :3]
-    for path in range(2):
+    for path in range(3):
         err = np.abs(got[path, : cfg.num_nodes, :3] - want).max() / np.abs(want).max()
         assert err <= 1e-12, (path, err)

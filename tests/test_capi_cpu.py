@@ -27,8 +27,10 @@ def lib():
 
 
 def test_every_declared_symbol_is_exported(lib):
-    hdr = open(os.path.join(ROOT, "include", "inductioneq.h")).read()
-    declared = set(re.findall(r"\b(ieq_\w+)\s*\(", hdr))
+    declared = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            declared |= set(re.findall(r"\b(ieq_\w+)\s*\(", open(os.path.join(ROOT, "include", h)).read()))
     assert declared == set(_capi.EXPORTS)
     for name in declared:
         assert hasattr(lib, name), name
@@ -118,6 +120,34 @@ def test_unknown_and_missing_kernels(lib):
         assert "kernel not in program" in lib.ieq_last_error(h).decode()
     names = (ctypes.c_char_p * 1)(b"SSPRK33_1")
     assert lib.ieq_run(h, names, 1, g, g, args, 2, None) == _capi.IEQ_ERR_ARGS  # needs 7 arguments
+    lib.ieq_program_release(h)
+
+
+def test_typed_entry_points_cover_every_reference_kernel():
+    """One ieq_<name> per kernel of the reference program (95), with the reference buffer count:
+    7 for the three-register integrators and their helpers, 6 for the 2N ones
+    (kernel_time_integrator.cl:224-227, 630-633, 1023-1026)."""
+    assert len(_capi.KERNEL_BUFFERS) == 95
+    assert _capi.KERNEL_BUFFERS["SSPRK33_1"] == 7 and _capi.KERNEL_BUFFERS["KennedyCarpenterLewis2R54C_5"] == 7
+    assert _capi.KERNEL_BUFFERS["CarpenterKennedy2N54_1a"] == 6 and _capi.KERNEL_BUFFERS["calc_curlB_rho_1_2args"] == 6
+    assert _capi.KERNEL_BUFFERS["calc_axpy"] == 3 and _capi.KERNEL_BUFFERS["init_b"] == 1
+
+
+def test_typed_entry_points_check_like_ieq_run(lib):
+    """ieq_<name> goes through the list interface: a kernel missing from the program, a NULL buffer and an
+    NDRange outside the grid give ieq_run's codes and messages, before anything is launched."""
+    cfg = ProgramConfig(testcase="rotation_2D", nx=8, ny=8, nz=8)  # no divergence-cleaning module
+    rc, h = _build(lib, cfg.settings_string(), cfg.modules())
+    assert rc == 0
+    g = (ctypes.c_uint32 * 3)(8, 8, 8)
+    big = (ctypes.c_uint32 * 3)(9, 8, 8)
+    p = [ctypes.c_void_p(64 * (k + 1)) for k in range(7)]  # never dereferenced: every call fails its checks first
+    assert lib.ieq_calc_laplace_divcleaning(h, g, None, p[0], p[1], None) == _capi.IEQ_ERR_KERNEL
+    assert "kernel not in program: calc_laplace_divcleaning" in lib.ieq_last_error(h).decode()
+    assert lib.ieq_SSPRK33_1(h, g, None, p[0], None, *p[2:], None) == _capi.IEQ_ERR_ARGS
+    assert "SSPRK33_1: argument 1 is NULL" in lib.ieq_last_error(h).decode()
+    assert lib.ieq_SSPRK33_1(h, big, None, *p, None) == _capi.IEQ_ERR_ARGS
+    assert "NDRange exceeds the grid" in lib.ieq_last_error(h).decode()
     lib.ieq_program_release(h)
 
 
