@@ -112,6 +112,23 @@ int ieq_download_real(ieq_program *prog, void *host, const void *src, size_t n);
 int ieq_run(ieq_program *prog, const char *const *names, int n, const uint32_t g[3], const uint32_t l[3], void *const *args,
             int nargs, double *kernel_seconds);
 
+/* A host array argument of ieq_run_host: `n` nodes of AoS REAL4 (vector != 0; 4*n REALs in the
+ * reference's host layout) or `n` REALs (vector == 0). */
+typedef struct {
+  void *host;
+  size_t n;
+  int vector;
+} ieq_host_arg;
+
+/* MatCL's run call on host arrays, with its copy semantics: every argument is copied to a
+ * device field (AoS REAL4 -> SoA), the kernel list runs as in ieq_run, and every argument is
+ * written back in place, as MatCL writes kernel arguments back into the MATLAB arrays
+ * (compute_numerical_solution.m:39-41,95-97; initialize.m:88-93).  kernel_seconds receives
+ * the kernels' time only (MatCL's return value), not the copies.  Device fields are
+ * allocated per call; for long runs keep fields on the device and use ieq_run. */
+int ieq_run_host(ieq_program *prog, const char *const *names, int n, const uint32_t g[3], const uint32_t l[3],
+                 const ieq_host_arg *args, int nargs, double *kernel_seconds);
+
 /* Advance `nsteps` full time steps of `integrator` (SSPRK33, SSPRK104,
  * KennedyCarpenterLewis2R54C, CalvoFrancoRandez2R64, CarpenterKennedy2N54,
  * ToulorgeDesmet2N84F) with the integrator's reference argument list

@@ -21,7 +21,8 @@ IEQ_ERR_UNSUPPORTED = -5
 EXPORTS = [
     "ieq_program_build", "ieq_program_release", "ieq_last_error", "ieq_program_get_info", "ieq_set_stream",
     "ieq_synchronize", "ieq_alloc", "ieq_free", "ieq_vector_field_bytes", "ieq_scalar_field_bytes",
-    "ieq_upload_real4", "ieq_download_real4", "ieq_upload_real", "ieq_download_real", "ieq_run", "ieq_step",
+    "ieq_upload_real4", "ieq_download_real4", "ieq_upload_real", "ieq_download_real", "ieq_run", "ieq_run_host",
+    "ieq_step",
     "ieq_clean_div_cg", "ieq_member_stride", "ieq_ensemble_create", "ieq_ensemble_step", "ieq_ensemble_config",
     "ieq_ensemble_release", "ieq_adjoint_create", "ieq_adjoint_forward", "ieq_adjoint_backward", "ieq_adjoint_counts",
     "ieq_adjoint_release",
@@ -72,6 +73,10 @@ EXPORTS += ["ieq_" + _n for _n in KERNEL_BUFFERS]
 
 _NOT_INT = ("ieq_program_release", "ieq_last_error", "ieq_vector_field_bytes", "ieq_scalar_field_bytes",
             "ieq_member_stride", "ieq_ensemble_release", "ieq_adjoint_release")
+
+
+class HostArg(ctypes.Structure):
+    _fields_ = [("host", ctypes.c_void_p), ("n", ctypes.c_size_t), ("vector", ctypes.c_int)]
 
 
 class CgResult(ctypes.Structure):
@@ -135,6 +140,8 @@ def load() -> ctypes.CDLL:
     u32x3 = ctypes.POINTER(ctypes.c_uint32)
     lib.ieq_run.argtypes = [P, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, u32x3, u32x3, ctypes.POINTER(P),
                             ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    lib.ieq_run_host.argtypes = [P, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, u32x3, u32x3, ctypes.POINTER(HostArg),
+                                 ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     lib.ieq_step.argtypes = [P, ctypes.c_char_p, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(P), ctypes.c_int,
                              ctypes.POINTER(ctypes.c_double)]
     lib.ieq_clean_div_cg.argtypes = [P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(CgResult)]

@@ -600,6 +600,33 @@ int ieq_run(ieq_program* prog, const char* const* names, int n, const uint32_t g
   return rc;
 }
 
+int ieq_run_host(ieq_program* prog, const char* const* names, int n, const uint32_t g[3], const uint32_t l[3],
+                 const ieq_host_arg* args, int nargs, double* kernel_seconds) {
+  if (!prog || !args || nargs <= 0) return IEQ_ERR_ARGS;
+  for (int i = 0; i < nargs; ++i) {
+    if (!args[i].host) return prog->fail(IEQ_ERR_ARGS, "host argument " + std::to_string(i) + " is NULL");
+    if ((long long)args[i].n > prog->info.stride)
+      return prog->fail(IEQ_ERR_ARGS, "host argument " + std::to_string(i) + " is larger than a field");
+  }
+  const size_t rb = prog->info.real_bytes;
+  std::vector<void*> dev(nargs, nullptr);
+  int rc = IEQ_OK;
+  for (int i = 0; i < nargs && rc == IEQ_OK; ++i) {  // copy in
+    const ieq_host_arg& h = args[i];
+    rc = ieq_alloc(prog, h.vector ? ieq_vector_field_bytes(prog) : std::max(ieq_scalar_field_bytes(prog), h.n * rb), &dev[i]);
+    if (rc == IEQ_OK) rc = h.vector ? ieq_upload_real4(prog, dev[i], h.host, h.n) : ieq_upload_real(prog, dev[i], h.host, h.n);
+  }
+  if (rc == IEQ_OK) rc = ieq_run(prog, names, n, g, l, dev.data(), nargs, kernel_seconds);
+  for (int i = 0; i < nargs && rc == IEQ_OK; ++i)  // write back in place
+    rc = args[i].vector ? ieq_download_real4(prog, args[i].host, dev[i], args[i].n)
+                        : ieq_download_real(prog, args[i].host, dev[i], args[i].n);
+  const std::string err = prog->err;
+  for (void* d : dev)
+    if (d) (void)ieq_free(prog, d);
+  prog->err = err;  // the first failure's message
+  return rc;
+}
+
 int ieq_step(ieq_program* prog, const char* integrator, long long nsteps, int variable_u, void* const* args, int nargs,
              double* kernel_seconds) {
   if (!prog || !integrator || nsteps < 0 || !args) return IEQ_ERR_ARGS;

@@ -143,6 +143,27 @@ class Program:
         _capi.check(rc, self.handle)
         return sec.value if timed else None
 
+    def run_host(self, names, arrays, g=None, l=None) -> float:
+        """MatCL's run call on host numpy arrays (ieq_run_host): (n, 4) arrays are REAL4 fields, 1-D arrays
+        REAL buffers; every array is written back in place.  Returns the kernel time in seconds."""
+        if isinstance(names, str):
+            names = [names]
+        g = g or (self.info.nx, self.info.ny, self.info.nz)
+        l = l or (self.info.w_size, 1, 1)
+        dt = np.float64 if self.info.real_bytes == 8 else np.float32
+        hargs = (_capi.HostArg * len(arrays))()
+        for i, a in enumerate(arrays):
+            if a.dtype != dt or not a.flags.c_contiguous or a.ndim not in (1, 2) or (a.ndim == 2 and a.shape[1] != 4):
+                raise ValueError("argument %d: need a C-contiguous %s array of shape (n, 4) or (n,)" % (i, dt.__name__))
+            hargs[i] = _capi.HostArg(a.ctypes.data, a.shape[0], int(a.ndim == 2))
+        G = (ctypes.c_uint32 * 3)(*[int(v) for v in g])
+        L = (ctypes.c_uint32 * 3)(*[int(v) for v in l])
+        cn = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+        sec = ctypes.c_double(0.0)
+        _capi.check(self.lib.ieq_run_host(self.handle, cn, len(names), G, L, hargs, len(arrays), ctypes.byref(sec)),
+                    self.handle)
+        return sec.value
+
     def step(self, integrator: str, nsteps: int, args, variable_u: bool = False, timed: bool = False):
         """Advance nsteps full time steps (fused kernels where the program qualifies)."""
         sec = ctypes.c_double(0.0)

@@ -151,6 +151,26 @@ def test_typed_entry_points_check_like_ieq_run(lib):
     lib.ieq_program_release(h)
 
 
+def test_run_host_checks_arguments_before_copying(lib):
+    """ieq_run_host (MatCL's copy-in / run / write-back call) rejects a NULL host array and one larger than a
+    field before it allocates or copies anything."""
+    import numpy as np
+
+    cfg = ProgramConfig(testcase="rotation_2D", nx=8, ny=8, nz=8)
+    rc, h = _build(lib, cfg.settings_string(), cfg.modules())
+    assert rc == 0
+    g = (ctypes.c_uint32 * 3)(8, 8, 8)
+    names = (ctypes.c_char_p * 1)(b"init_b")
+    a = np.zeros((cfg.num_nodes_pad, 4))
+    args = (_capi.HostArg * 1)(_capi.HostArg(None, cfg.num_nodes_pad, 1))
+    assert lib.ieq_run_host(h, names, 1, g, g, args, 1, None) == _capi.IEQ_ERR_ARGS
+    assert "host argument 0 is NULL" in lib.ieq_last_error(h).decode()
+    args = (_capi.HostArg * 1)(_capi.HostArg(a.ctypes.data, 1 << 40, 1))
+    assert lib.ieq_run_host(h, names, 1, g, g, args, 1, None) == _capi.IEQ_ERR_ARGS
+    assert "larger than a field" in lib.ieq_last_error(h).decode()
+    lib.ieq_program_release(h)
+
+
 def test_member_stride(lib):
     assert lib.ieq_member_stride(10, 10, 10) == 1024
     assert lib.ieq_member_stride(8, 8, 1) == 64
