@@ -106,14 +106,34 @@ def initialize(cfg: ProgramConfig, make_executor: Callable[[ProgramConfig], obje
 
 def compute_numerical_solution(cfg: ProgramConfig, ex, integrator: str, b0: np.ndarray, u0: np.ndarray, rho0: np.ndarray,
                                num_steps: int, final_time: float, variable_u: bool = False, fused: bool = False,
-                               analytic: bool = True) -> Dict[str, object]:
-    """compute_numerical_solution.m: advance num_steps, then error / divergence / energy."""
+                               analytic: bool = True, save_integrals_over_time: bool = False) -> Dict[str, object]:
+    """compute_numerical_solution.m: advance num_steps, then error / divergence / energy.
+
+    save_integrals_over_time (I_RunOps('save_integrals_over_time'), compute_numerical_solution.m:15-19,
+    69-89, 218-250): before every step record the energy (norm2 of b), the divergence norm
+    (calc_div + norm2_S) and the error against analytical_b at the current time (written into the
+    second field register, as the reference does, and compared with norm2_diff); the final values
+    complete the series, returned as energy_over_time, L2error_B_over_time, L2error_divB_over_time
+    (num_steps + 1 entries) and time = linspace(0, final_time, num_steps + 1)."""
     names, nf = rk_schedule(integrator, cfg.hall, variable_u)
     fields = [ex.dev_vec(b0)] + [ex.vec() for _ in range(nf - 1)]
     J, u, rho, tb = ex.vec(), ex.dev_vec(u0), ex.dev_scal(rho0), ex.time(0.0)
     args = fields + [J, u, rho, tb]
+    series = {"energy": [], "L2error_B": [], "L2error_divB": []}
     t0 = _time.perf_counter()
-    if fused:
+    if save_integrals_over_time:
+        div_t = ex.scal()
+        for _ in range(num_steps):
+            series["energy"].append(ex.reduce("norm2", fields[0]))
+            ex.run("calc_div", [fields[0], div_t])
+            series["L2error_divB"].append(math.sqrt(max(ex.reduce("norm2_S", div_t), 0.0)))
+            ex.run("analytical_b", [fields[1], tb])
+            series["L2error_B"].append(math.sqrt(max(ex.reduce("norm2_diff", fields[0], fields[1]), 0.0)))
+            if fused:
+                ex.step(integrator, 1, args, variable_u=variable_u)
+            else:
+                ex.run(names, args)
+    elif fused:
         ex.step(integrator, num_steps, args, variable_u=variable_u)
     else:
         block = 15000  # RK_block_size, compute_numerical_solution.m:14
@@ -136,6 +156,12 @@ def compute_numerical_solution(cfg: ProgramConfig, ex, integrator: str, b0: np.n
     ex.run("calc_div", [fields[0], div])
     res["divergence_norm"] = math.sqrt(max(ex.reduce("norm2_S", div), 0.0))
     res["energy"] = ex.reduce("norm2", fields[0])
+    if save_integrals_over_time:
+        res["energy_over_time"] = np.array(series["energy"] + [res["energy"]])
+        res["L2error_divB_over_time"] = np.array(series["L2error_divB"] + [res["divergence_norm"]])
+        if analytic:
+            res["L2error_B_over_time"] = np.array(series["L2error_B"] + [res["abs_err"]])
+        res["time_series"] = np.linspace(0.0, final_time, num_steps + 1)
     return res
 
 

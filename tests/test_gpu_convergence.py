@@ -61,3 +61,27 @@ def test_rotation_2D_converges(ops, rate):
         assert abs(g - o) <= 1e-10, (g, o)  # errors relative to the analytic field's norm
     got = math.log(g1 / g2, 30 / 15)
     assert got > rate, (g1, g2, got)
+
+
+def test_integrals_over_time_match_oracle():
+    """save_integrals_over_time through the HIP library: the per-step energy, divergence-norm and
+    error series (compute_numerical_solution.m:69-89) equal the oracle's."""
+    import numpy as np
+    import torch
+    from pyoracle import Oracle
+
+    L = 4 * math.pi / 3
+    out = []
+    for make in (HipExecutor, Oracle):
+        cfg = ProgramConfig(opset="4thOrder", nx=12, ny=12, nz=12, xmin=0, xmax=L, ymin=0, ymax=L, zmin=0, zmax=L,
+                            periodic=True, hall=True, testcase="hall_periodic", w_size=16)
+        init = initialize(cfg, make, 0.2, 0.05)
+        cfg.dt = init["dt"]
+        out.append(compute_numerical_solution(cfg, make(cfg), "CarpenterKennedy2N54", init["b"], init["u"], init["rho"],
+                                              init["num_steps"], 0.05, variable_u=True, save_integrals_over_time=True))
+    torch.cuda.synchronize()
+    g, o = out
+    scale = math.sqrt(o["energy_over_time"][0])
+    for k in ("energy_over_time", "L2error_B_over_time", "L2error_divB_over_time"):
+        assert g[k].shape == o[k].shape
+        assert np.allclose(g[k], o[k], rtol=1e-10, atol=1e-12 * scale), k
