@@ -85,3 +85,24 @@ def test_integrals_over_time_match_oracle():
     for k in ("energy_over_time", "L2error_B_over_time", "L2error_divB_over_time"):
         assert g[k].shape == o[k].shape
         assert np.allclose(g[k], o[k], rtol=1e-10, atol=1e-12 * scale), k
+
+
+def test_convergence_sweep_on_gpu_matches_oracle():
+    """The paper-style sweep (inductioneq_amd.sweeps) on the HIP library: every table entry (energy, error in B,
+    error in div B) equals the oracle's, and the rates are the operator orders."""
+    import torch
+    from pyoracle import Oracle
+
+    from inductioneq_amd.sweeps import FORMS_PAPER, convergence_sweep, observed_orders
+
+    forms = [FORMS_PAPER[0], FORMS_PAPER[3]]
+    got = convergence_sweep((12, 24), (2, 4, 6), forms, final_time=0.05)
+    want = convergence_sweep((12, 24), (2, 4, 6), forms, final_time=0.05, make_executor=Oracle)
+    torch.cuda.synchronize()
+    for g, w in zip(got, want):
+        scale = math.sqrt(w["result"]["energy"])  # the field's M-norm: errors and divergence agree to 1e-12 of it
+        assert abs(g["result"]["energy"] - w["result"]["energy"]) <= 1e-10 * w["result"]["energy"], (g["N"], g["order"])
+        for k in ("abs_err", "divergence_norm"):
+            assert abs(g["result"][k] - w["result"][k]) <= 1e-12 * scale, (g["N"], g["order"], k)
+    for (order, _), (rate,) in observed_orders(got).items():
+        assert rate > min(order, 4) - 0.3, (order, rate)

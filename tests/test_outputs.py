@@ -70,3 +70,21 @@ def test_table_and_time_series_files(tmp_path):
             got = {k: f[k][()] for k in f.keys()}
     assert sorted(got) == ["L2error_B", "L2error_divB", "energy", "time"]
     assert np.array_equal(got["energy"], res["energy_over_time"]) and np.array_equal(got["time"], res["time_series"])
+
+
+def test_convergence_sweep_table(tmp_path):
+    """convergence_sweep (paper/convergence_hall_periodic.m:99-152) on the oracle executor: one table line per
+    (N, order, forms), in the reference's loop order, and the hall_periodic rates of the operator orders."""
+    from pyoracle import Oracle
+
+    from inductioneq_amd.sweeps import FORMS_PAPER, convergence_sweep, observed_orders
+
+    forms = [FORMS_PAPER[1], FORMS_PAPER[5]]
+    path = tmp_path / "hall_periodic_convergence.txt"
+    runs = convergence_sweep((12, 24), (2, 4), forms, final_time=0.05, make_executor=Oracle, path=str(path))
+    rows = read_convergence_table(str(path))
+    assert [(r[0], r[1], r[2]) for r in rows] == [(n, o, "USE_UIBJ_" + f[0]) for n in (12, 24) for o in (2, 4)
+                                                  for f in forms]
+    rates = observed_orders(runs)
+    for (order, _), (rate,) in rates.items():
+        assert rate > order - 0.2, (order, rate)
