@@ -403,6 +403,8 @@ int ensemble_step(Ensemble* e, long long nsteps, double* seconds, std::string& e
   if (!e->tuned) {
     rc = e->real_bytes == 8 ? ens_tune<double>(*e, err) : ens_tune<float>(*e, err);
     if (rc) return IEQ_ERR_HIP;
+    // IEQ_ENS_GRAPH=1 / 0 overrides the timed choice between graph replay and direct launches
+    if (const char* g = getenv("IEQ_ENS_GRAPH")) e->use_graph = atoi(g) != 0;
   }
   if (nsteps % e->steps_per_graph) return err = "this integrator advances the ensemble in pairs of steps", IEQ_ERR_ARGS;
   hipEvent_t t0, t1;

@@ -119,3 +119,24 @@ def test_ensemble_fp32_within_1e5_of_fp64():
         a, b = e64.state(i).numpy(), e32.state(i).numpy().astype(np.float64)
         err = np.abs(a - b).max() / np.abs(a).max()
         assert err <= 1e-5, "member %d: %.3e" % (i, err)
+
+
+@pytest.mark.parametrize("integrator,nrep", [("SSPRK33", 4), ("CarpenterKennedy2N54", 3)])
+def test_graph_replays_each_match_oracle(integrator, nrep, monkeypatch):
+    """The captured step forced into graph replay (IEQ_ENS_GRAPH=1): every replay, with eager device work and
+    host read-backs between replays, equals the oracle run for that many steps (FP64, 1e-12)."""
+    import torch
+
+    monkeypatch.setenv("IEQ_ENS_GRAPH", "1")
+    cfgs, ens, Bs = _ensemble("6thOrder", True, "HIGH_ORDER", integrator, "double", seeds=(7, 8, 9))
+    per = 2 if integrator == "CarpenterKennedy2N54" else 1  # steps per captured graph
+    for rep in range(1, nrep + 1):
+        ens.step(per)
+        assert ens.config()["graph"]
+        junk = torch.randn(1 << 20, device="cuda", dtype=torch.float64)  # eager work between replays
+        assert torch.isfinite(junk.sum()).item()
+        torch.cuda.synchronize()
+        for i, (cfg, B) in enumerate(zip(cfgs, Bs)):
+            want = _oracle_steps(cfg, integrator, B, MEMBERS[i][3], rep * per)
+            got = ens.state(i).numpy()
+            assert np.abs(got - want).max() / np.abs(want).max() <= 1e-12, (rep, i)
