@@ -168,6 +168,23 @@ def main():
     finite = bool(torch.isfinite(b1[:, : cfg.num_nodes]).all().item())
     print("bench: %.3f ms for %d steps, %.1f us/stage, finite=%s" % (ms, a.steps, stage_s * 1e6, finite), file=sys.stderr,
           flush=True)
+    err64 = None
+    if real == "float":
+        # the FP32 state against the FP64 path (equal to the CPU oracle to 1e-12 per step) after the same steps
+        from dataclasses import replace
+
+        p64 = Program(replace(cfg, real="double"))
+        p64.use_current_stream()
+        c1 = p64.vector()
+        c1[:, : cfg.num_nodes] = torch.as_tensor(B0.T, dtype=torch.float64, device="cuda")
+        p64.step(a.integrator, a.warmup + a.steps, [c1, p64.vector(), p64.vector(), p64.vector(), p64.scalar(1.0),
+                                                    p64.time_buffer()])
+        torch.cuda.synchronize()
+        ref = c1[:, : cfg.num_nodes]
+        err64 = float(((b1[:, : cfg.num_nodes].double() - ref).abs().max() / ref.abs().max()).item())
+        print("bench: fp32 vs fp64 after %d steps: max rel err %.3e" % (a.warmup + a.steps, err64), file=sys.stderr,
+              flush=True)
+        del p64, c1, ref
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -200,6 +217,8 @@ def main():
                      "us_per_stage": round(stage_s * 1e6, 2)},
         "cpu_baseline": cpu,
     }
+    if err64 is not None:
+        out["max_rel_err_vs_fp64"] = err64
     print(json.dumps(out), flush=True)
 
 
