@@ -44,7 +44,7 @@ constexpr int kLapRows = 32;  // table rows held in LDS by k_cg_lap (2 * NUM_BOU
 // Deterministic block reduction of one double (wave64 shuffles, then LDS).
 __device__ inline double block_sum(double v, double* smem) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int lane = threadIdx.x & 63, wid = (threadIdx.y * blockDim.x + threadIdx.x) >> 6;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int nw = (blockDim.x * blockDim.y + 63) >> 6;
   if (lane == 0) smem[wid] = v;
   __syncthreads();
@@ -83,8 +83,8 @@ struct LapArgs {
   unsigned int nblocks;
 };
 
-// Minimum waves per SIMD the register allocation of a 16-row Laplacian tile must allow.  A
-// workgroup of 64 x TY threads puts TY/4 waves on each SIMD, so it needs at least 4 (the
+// Minimum waves per SIMD the register allocation of a 1024-thread Laplacian tile must allow.  A
+// workgroup of 64 x 16 threads puts 4 waves on each SIMD, so it needs at least 4 (the
 // unconstrained FP64 kernel at half-width 4 takes 133 VGPRs, 3 waves per SIMD, and would not
 // launch).  With one plane of loads in flight the kernels up to half-width 5 fit without
 // spilling (half-width 4: FP64 120 VGPRs, 4 waves; FP32 74 VGPRs, 6 waves).
@@ -108,11 +108,11 @@ struct LapArgs {
 // load is waited for before it is needed.  The per-plane barrier orders LDS traffic only.
 // Tiles are handed out per XCD in contiguous ranges (x fastest, then y, then z-chunk), so
 // the halo rows of y-neighbours are read through one L2.
-template <typename R, int HW, int TY, int PD>
-__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY >= 16 ? (sizeof(R) == 4 ? IEQ_LAP_WAVES32 : IEQ_LAP_WAVES64) : 1, 8))) void k_cg_lap(LapArgs a, const R* __restrict__ u, const R* __restrict__ r,
+template <typename R, int HW, int TY, int PD, int TX>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(TX * TY >= 1024 ? (sizeof(R) == 4 ? IEQ_LAP_WAVES32 : IEQ_LAP_WAVES64) : 1, 8))) void k_cg_lap(LapArgs a, const R* __restrict__ u, const R* __restrict__ r,
                                                     R* __restrict__ un, R* __restrict__ q, CgState* st,
                                                     const double* rr_part, unsigned int n_rr, double* part, int iter) {
-  constexpr int LX = 64 + 2 * HW, LY = TY + 2 * HW, Q = 2 * HW + 1;
+  constexpr int LX = TX + 2 * HW, LY = TY + 2 * HW, Q = 2 * HW + 1;
   __shared__ R s[2][LY][LX];
   __shared__ double red[17];
   if (st->done) return;
@@ -133,10 +133,10 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY >= 1
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int gx = a.nx - 2 * a.off, gy = a.ny - 2 * a.off, gz = a.nz - 2 * a.off;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int ntx = (gx + 63) / 64, nty = (gy + TY - 1) / TY;
+  const int ntx = (gx + TX - 1) / TX, nty = (gy + TY - 1) / TY;
   const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
-  const int x = a.off + bx * 64 + tx, y = a.off + by * TY + ty;
-  const int x0 = a.off + bx * 64, y0 = a.off + by * TY;
+  const int x = a.off + bx * TX + tx, y = a.off + by * TY + ty;
+  const int x0 = a.off + bx * TX, y0 = a.off + by * TY;
   const bool active = (x - a.off) < gx && (y - a.off) < gy;
   const unsigned RB = sizeof(R);
   const unsigned plb = (unsigned)(a.nx * a.ny) * RB;
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY >= 1
   constexpr int NYH = 2 * HW <= TY ? 1 : 2;
   const unsigned oc = (unsigned)(cl(x, a.nx) + a.nx * cl(y, a.ny)) * RB;
   const bool colin = inb(x, a.nx) && inb(y, a.ny);
-  const bool hl = tx < HW, hr = tx >= 64 - HW, hb = ty < HW, ht = ty >= TY - HW;
+  const bool hl = tx < HW, hr = tx >= TX - HW, hb = ty < HW, ht = ty >= TY - HW;
   const int hxx = hl ? x0 - HW + tx : x + HW;  // x-halo cell (used when hl || hr)
   const int hyb = y0 - HW + ty, hya = y + HW;    // y-halo cells below / above
   const unsigned ohx = (hl || hr) ? (unsigned)(cl(hxx, a.nx) + a.nx * cl(y, a.ny)) * RB : oc;
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY >= 1
   __shared__ R sz[kLapRows][Q];
   {
     const int rows = 2 * a.nbl + 1, c0 = (a.wl - 1) / 2;
-    for (int i = ty * 64 + tx; i < rows * Q; i += 64 * TY) {
+    for (int i = ty * TX + tx; i < rows * Q; i += TX * TY) {
       const int row = i / Q, k = i % Q, j = c0 - HW + k;
       sz[row][k] = (j >= 0 && j < a.wl) ? (R)(a.tab[row * a.wl + j] * a.idz2) : R(0);
     }
@@ -363,28 +363,34 @@ constexpr int kLapPD = IEQ_LAP_PD;  // planes of the Laplacian's z-march in flig
 #ifndef IEQ_LAP_TY
 #define IEQ_LAP_TY 16
 #endif
-// Rows of a Laplacian tile (64 x TY threads) for stencil half-width hw: 16 up to hw = 5; the
-// wider stencils hold more registers (they would spill at 4 waves per SIMD) and run 8 rows.
+#ifndef IEQ_LAP_TX
+#define IEQ_LAP_TX 64
+#endif
+// Shape of a Laplacian tile (TX x TY threads) for stencil half-width hw: 64 x 16 up to hw = 5;
+// the wider stencils hold more registers (they would spill at 4 waves per SIMD) and run 64 x 8.
 constexpr int lap_ty(int hw) { return hw <= 5 ? IEQ_LAP_TY : 8; }
+constexpr int lap_tx(int hw) { return hw <= 5 ? IEQ_LAP_TX : 64; }
 
 template <typename R, int HW, int TY>
 void launch_lap(const LapArgs& a, const R* u, const R* r, R* un, R* q, CgState* st, const double* rr_part, unsigned int n_rr,
                 double* part, int iter, hipStream_t s) {
   const int gx = a.nx - 2 * a.off, gy = a.ny - 2 * a.off, gz = a.nz - 2 * a.off;
-  const int grid = ((gx + 63) / 64) * ((gy + TY - 1) / TY) * ((gz + a.zc - 1) / a.zc);
-  k_cg_lap<R, HW, TY, kLapPD><<<grid, dim3(64, TY), 0, s>>>(a, u, r, un, q, st, rr_part, n_rr, part, iter);
+  constexpr int TX = lap_tx(HW);
+  const int grid = ((gx + TX - 1) / TX) * ((gy + TY - 1) / TY) * ((gz + a.zc - 1) / a.zc);
+  k_cg_lap<R, HW, TY, kLapPD, TX><<<grid, dim3(TX, TY), 0, s>>>(a, u, r, un, q, st, rr_part, n_rr, part, iter);
 }
 
 // Planes per z-chunk: the smallest chunk whose grid fits in one round of resident
 // workgroups (so no partial second round), at least 8 planes, rounded up to the kernel's
 // group of PD+1 planes.
 template <typename R, int HW, int TY> int lap_zc(int gx, int gy, int gz) {
+  constexpr int TX = lap_tx(HW);
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_lap<R, HW, TY, kLapPD>, 64 * TY, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_lap<R, HW, TY, kLapPD, TX>, TX * TY, 0) != hipSuccess ||
       cus <= 0 || per_cu <= 0)
     return 16;
-  const long long xy = (long long)((gx + 63) / 64) * ((gy + TY - 1) / TY);
+  const long long xy = (long long)((gx + TX - 1) / TX) * ((gy + TY - 1) / TY);
   const long long resident = (long long)cus * per_cu;
   const long long chunks = std::max(1LL, resident / std::max(1LL, xy));
   const int ns = kLapPD + 1;  // the kernel's z-loop advances in groups of ns planes
@@ -425,7 +431,8 @@ int cg_run_t(const Params& p, const ieq_program_info& info, const double* host_t
     case 6: zc = lap_zc<R, 6, lap_ty(6)>(gx, gy, gz); break;
     default: zc = lap_zc<R, 7, lap_ty(7)>(gx, gy, gz); break;
   }
-  const unsigned int lap_blocks = (unsigned)(((gx + 63) / 64) * ((gy + TY - 1) / TY) * ((gz + zc - 1) / zc));
+  const int TX = lap_tx(hw);
+  const unsigned int lap_blocks = (unsigned)(((gx + TX - 1) / TX) * ((gy + TY - 1) / TY) * ((gz + zc - 1) / zc));
   const unsigned int vec_blocks = 1024;
   const size_t fbytes = ((size_t)info.stride * sizeof(R) + 255) / 256 * 256;
   const size_t part_bytes = (size_t)(lap_blocks + vec_blocks) * sizeof(double);  // u'.q partials, then r.r partials
