@@ -108,7 +108,12 @@ int ieq_download_real(ieq_program *prog, void *host, const void *src, size_t n);
  * kernel_div_cleaning.cl, kernel_CG.cl and kernel_dot_product.cl.  `g` is the NDRange,
  * `l` the local size (only the reductions use it: one output per l[0] nodes).
  * Scalars such as the CG coefficient are 1-element device buffers (kernel_CG.cl:6-7).
- * When kernel_seconds is non-NULL the call waits for the kernels and stores their time. */
+ * When kernel_seconds is non-NULL the call waits for the kernels and stores their time.
+ * A list that is one integrator's RK_Step repeated k times over the whole grid (the
+ * reference time loop's repmat(RK_Step, 1, k), compute_numerical_solution.m:91-108)
+ * runs as ieq_step(integrator, k) would: on a periodic grid with uniform u and rho the
+ * fused stage kernels advance b1, the other field registers, curlB_rho and time to the
+ * same state as the listed kernels.  ieq_last_step_path reports which path ran. */
 int ieq_run(ieq_program *prog, const char *const *names, int n, const uint32_t g[3], const uint32_t l[3], void *const *args,
             int nargs, double *kernel_seconds);
 
@@ -136,11 +141,24 @@ int ieq_run_host(ieq_program *prog, const char *const *names, int n, const uint3
  * RK_Step list of compute_numerical_solution_setup.m:140-371 (block loop of
  * compute_numerical_solution.m:91-108); when the grid is periodic and u and rho are
  * uniform it runs the fused stage kernels instead of the reference kernel sequence.
+ * Uniformity is checked on the device, which costs one host synchronisation per call.
  * On return b1 and time hold the same state as after the reference schedule; b2/b3
  * and curlB_rho are work registers whose contents are unspecified.
  * `variable_u` requests calc_u before each stage (I_RunOps('variable_u')). */
 int ieq_step(ieq_program *prog, const char *integrator, long long nsteps, int variable_u, void *const *args, int nargs,
              double *kernel_seconds);
+
+/* ieq_step for a caller that knows u and rho are uniform: the fields hold u[0..2] and rho at
+ * every node, and these values are used without reading the fields, so the call enqueues
+ * its kernels without any host synchronisation.  Same results as ieq_step on such fields.
+ * No counterpart in the reference (its coefficients are always fields). */
+int ieq_step_uniform(ieq_program *prog, const char *integrator, long long nsteps, const double u[3], double rho,
+                     void *const *args, int nargs, double *kernel_seconds);
+
+/* Path taken by the last successful ieq_run / ieq_step / ieq_step_uniform on prog: 1 the fused
+ * stage kernels, 0 the reference kernel sequence, -1 none (or the call failed).  Setting
+ * IEQ_NO_FUSED=1 in the environment keeps every call on the reference kernel sequence. */
+int ieq_last_step_path(const ieq_program *prog);
 
 /* Result of ieq_clean_div_cg. */
 typedef struct {

@@ -169,6 +169,7 @@ struct ieq_program {
   std::string err;
   ieq::FusedWork* fused = nullptr;
   ieq::CgWork* cg = nullptr;
+  int last_path = -1;  // path of the last ieq_run / ieq_step: 0 reference kernels, 1 fused kernels
 
   bool has(const char* m) const { return modules.count(m) != 0; }
   int fail(int code, const std::string& msg) {
@@ -468,6 +469,72 @@ int run_list(ieq_program* pr, const char* const* names, int n, const uint32_t g[
   return IEQ_OK;
 }
 
+// Kernel time of one ABI call (hip events on the program's stream); destroys its events on
+// every path.
+struct KernelTimer {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int begin(ieq_program* pr, bool on) {
+    if (!on) return IEQ_OK;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, pr->stream);
+    if (e != hipSuccess) {
+      release();
+      return hip_fail(pr, e, "kernel timer");
+    }
+    return IEQ_OK;
+  }
+  void end(ieq_program* pr, int rc, double* seconds) {
+    if (!seconds || !e0) return;
+    float ms = 0;
+    if (hipEventRecord(e1, pr->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      ms = 0;
+    *seconds = rc ? 0.0 : ms * 1e-3;
+    release();
+  }
+  void release() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    e0 = e1 = nullptr;
+  }
+  ~KernelTimer() { release(); }
+};
+
+// IEQ_NO_FUSED=1 in the environment keeps every call on the reference kernel sequence.
+bool fused_allowed() {
+  const char* v = std::getenv("IEQ_NO_FUSED");
+  return !(v && *v && std::strcmp(v, "0") != 0);
+}
+
+bool full_grid(const ieq_program* pr, const uint32_t g[3]) {
+  return g[0] == (uint32_t)pr->info.nx && g[1] == (uint32_t)pr->info.ny && g[2] == (uint32_t)pr->info.nz;
+}
+
+// Is names[0..n) the RK_Step list of one integrator (compute_numerical_solution_setup.m:140-371, without
+// calc_u) repeated k >= 1 times, with that integrator's argument count?
+bool match_rk_list(const ieq_program* pr, const char* const* names, int n, int nargs, const char** integ, long long* k) {
+  static const char* kIntegrators[6] = {"SSPRK33", "SSPRK104", "KennedyCarpenterLewis2R54C", "CalvoFrancoRandez2R64",
+                                        "CarpenterKennedy2N54", "ToulorgeDesmet2N84F"};
+  if (n <= 0 || !names[0]) return false;
+  for (const char* I : kIntegrators) {
+    if (std::strncmp(names[0], "calc_curlB_rho", 14) && std::strncmp(names[0], I, std::strlen(I))) continue;
+    std::vector<std::string> sched;
+    int nf = 0;
+    if (ieq::schedule(I, pr->info.hall, 0, sched, nf) || nargs != nf + 4) continue;
+    const int m = (int)sched.size();
+    if (n % m) continue;
+    bool ok = true;
+    for (int i = 0; i < n && ok; ++i) ok = names[i] && sched[i % m] == names[i];
+    if (ok) {
+      *integ = I;
+      *k = n / m;
+      return true;
+    }
+  }
+  return false;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------
@@ -581,22 +648,22 @@ int ieq_download_real(ieq_program* prog, void* host, const void* src, size_t n) 
 int ieq_run(ieq_program* prog, const char* const* names, int n, const uint32_t g[3], const uint32_t l[3], void* const* args,
             int nargs, double* kernel_seconds) {
   if (!prog || (!names && n > 0) || !g || (!args && nargs > 0) || n < 0) return IEQ_ERR_ARGS;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (kernel_seconds) {
-    HIP_TRY(prog, hipEventCreate(&e0));
-    HIP_TRY(prog, hipEventCreate(&e1));
-    HIP_TRY(prog, hipEventRecord(e0, prog->stream));
+  KernelTimer tm;
+  int rc = tm.begin(prog, kernel_seconds != nullptr);
+  if (rc) return rc;
+  int used = 0;
+  const char* integ = nullptr;
+  long long k = 0;
+  // The reference's time loop runs repmat(RK_Step, 1, k) in one call (compute_numerical_solution.m:91-108):
+  // such a list over the whole grid takes the fused kernels when the program qualifies.
+  if (fused_allowed() && full_grid(prog, g) && match_rk_list(prog, names, n, nargs, &integ, &k)) {
+    ieq::FusedOpts opt;
+    opt.final_curl = true;  // every argument ends as after the listed kernels
+    rc = ieq::fused_step(prog->p, prog->info, prog->stream, &prog->fused, integ, k, 0, args, opt, &used, prog->err);
   }
-  int rc = run_list(prog, names, n, g, l, args, nargs);
-  if (kernel_seconds) {
-    (void)hipEventRecord(e1, prog->stream);
-    (void)hipEventSynchronize(e1);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    *kernel_seconds = rc ? 0.0 : ms * 1e-3;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-  }
+  if (rc == IEQ_OK && !used) rc = run_list(prog, names, n, g, l, args, nargs);
+  prog->last_path = rc == IEQ_OK ? used : -1;
+  tm.end(prog, rc, kernel_seconds);
   return rc;
 }
 
@@ -627,23 +694,23 @@ int ieq_run_host(ieq_program* prog, const char* const* names, int n, const uint3
   return rc;
 }
 
-int ieq_step(ieq_program* prog, const char* integrator, long long nsteps, int variable_u, void* const* args, int nargs,
-             double* kernel_seconds) {
+static int step_impl(ieq_program* prog, const char* integrator, long long nsteps, int variable_u, void* const* args,
+                     int nargs, const double* uniform, double* kernel_seconds) {
   if (!prog || !integrator || nsteps < 0 || !args) return IEQ_ERR_ARGS;
   std::vector<std::string> sched;
   int nf = 0;
   int rc = ieq::schedule(integrator, prog->info.hall, variable_u, sched, nf);
   if (rc) return prog->fail(IEQ_ERR_ARGS, std::string("Unkown time integrator ") + integrator + "!");
   if (nargs != nf + 4) return prog->fail(IEQ_ERR_ARGS, std::string(integrator) + " steps take " + std::to_string(nf + 4) + " arguments");
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (kernel_seconds) {
-    HIP_TRY(prog, hipEventCreate(&e0));
-    HIP_TRY(prog, hipEventCreate(&e1));
-    HIP_TRY(prog, hipEventRecord(e0, prog->stream));
-  }
+  KernelTimer tm;
+  if ((rc = tm.begin(prog, kernel_seconds != nullptr))) return rc;
   int used_fused = 0;
-  rc = ieq::fused_step(prog->p, prog->info, prog->stream, &prog->fused, integrator, nsteps, variable_u, args, &used_fused,
-                       prog->err);
+  if (fused_allowed()) {
+    ieq::FusedOpts opt;
+    opt.uniform = uniform;
+    rc = ieq::fused_step(prog->p, prog->info, prog->stream, &prog->fused, integrator, nsteps, variable_u, args, opt,
+                         &used_fused, prog->err);
+  }
   if (rc == IEQ_OK && !used_fused) {
     std::vector<const char*> names;
     for (auto& s : sched) names.push_back(s.c_str());
@@ -651,17 +718,26 @@ int ieq_step(ieq_program* prog, const char* integrator, long long nsteps, int va
     uint32_t l[3] = {(uint32_t)prog->info.w_size, 1, 1};
     for (long long k = 0; k < nsteps && rc == IEQ_OK; ++k) rc = run_list(prog, names.data(), (int)names.size(), g, l, args, nargs);
   }
-  if (kernel_seconds) {
-    (void)hipEventRecord(e1, prog->stream);
-    (void)hipEventSynchronize(e1);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    *kernel_seconds = rc ? 0.0 : ms * 1e-3;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-  }
+  prog->last_path = rc == IEQ_OK ? used_fused : -1;
+  tm.end(prog, rc, kernel_seconds);
   return rc;
 }
+
+int ieq_step(ieq_program* prog, const char* integrator, long long nsteps, int variable_u, void* const* args, int nargs,
+             double* kernel_seconds) {
+  return step_impl(prog, integrator, nsteps, variable_u, args, nargs, nullptr, kernel_seconds);
+}
+
+int ieq_step_uniform(ieq_program* prog, const char* integrator, long long nsteps, const double u[3], double rho,
+                     void* const* args, int nargs, double* kernel_seconds) {
+  if (!u) return IEQ_ERR_ARGS;
+  const double uni[4] = {u[0], u[1], u[2], rho};
+  for (double v : uni)
+    if (!std::isfinite(v)) return prog ? prog->fail(IEQ_ERR_ARGS, "uniform u and rho must be finite") : IEQ_ERR_ARGS;
+  return step_impl(prog, integrator, nsteps, 0, args, nargs, uni, kernel_seconds);
+}
+
+int ieq_last_step_path(const ieq_program* prog) { return prog ? prog->last_path : -1; }
 
 }  // extern "C"
 

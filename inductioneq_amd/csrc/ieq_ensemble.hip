@@ -254,25 +254,49 @@ size_t member_bytes(const Ensemble& e, const ieq_member& mb) {
   return (size_t)3 * ((((size_t)mb.nx * mb.ny * mb.nz) + 63) / 64 * 64) * e.real_bytes;
 }
 
+// Member registers saved around the tuner's trial steps.  Every exit path of the tuner restores
+// them (the trial steps advance the members), frees the snapshot and destroys the events.
+struct TuneScope {
+  Ensemble& e;
+  char* save = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  explicit TuneScope(Ensemble& en) : e(en) {}
+  void copy(bool restore) {
+    size_t off = 0;
+    for (auto& mb : e.m) {
+      void* f[3] = {mb.b1, mb.b2, mb.b3};
+      for (int j = 0; j < 3; ++j) {
+        const size_t nb = member_bytes(e, mb);
+        if (restore) (void)hipMemcpyAsync(f[j], save + off, nb, hipMemcpyDeviceToDevice, e.stream);
+        else (void)hipMemcpyAsync(save + off, f[j], nb, hipMemcpyDeviceToDevice, e.stream);
+        off += nb;
+      }
+    }
+  }
+  ~TuneScope() {
+    if (save) {
+      (void)hipStreamSynchronize(e.stream);  // trial launches done before the restore
+      copy(true);
+      (void)hipStreamSynchronize(e.stream);
+      (void)hipFree(save);
+    }
+    for (hipEvent_t& v : ev)
+      if (v) (void)hipEventDestroy(v);
+  }
+};
+
 template <typename R> int ens_tune(Ensemble& e, std::string& err) {
   // save the registers the trial steps overwrite
   size_t total = 0;
   for (auto& mb : e.m) total += 3 * member_bytes(e, mb);
-  char* save = nullptr;
-  if (hipMalloc(&save, total) != hipSuccess) return err = "hipMalloc failed (tuning snapshot)", -4;
-  size_t off = 0;
-  for (auto& mb : e.m) {
-    void* f[3] = {mb.b1, mb.b2, mb.b3};
-    for (int j = 0; j < 3; ++j) {
-      (void)hipMemcpyAsync(save + off, f[j], member_bytes(e, mb), hipMemcpyDeviceToDevice, e.stream);
-      off += member_bytes(e, mb);
-    }
-  }
+  TuneScope sc(e);
+  if (hipMalloc(&sc.save, total) != hipSuccess) return sc.save = nullptr, err = "hipMalloc failed (tuning snapshot)", -4;
+  sc.copy(false);
+  for (hipEvent_t& v : sc.ev)
+    if (hipEventCreate(&v) != hipSuccess) return v = nullptr, err = "event creation failed", -4;
+  hipEvent_t t0 = sc.ev[0], t1 = sc.ev[1], t2 = sc.ev[2];
   float best = 1e30f;
   int bty = 8, bzc = 8, bord = 1;
-  hipEvent_t t0, t1;
-  (void)hipEventCreate(&t0);
-  (void)hipEventCreate(&t1);
   const int tys[3] = {4, 8, 32}, zcs[3] = {4, 8, 16};
   for (int ty : tys)
     for (int zc : zcs)
@@ -299,53 +323,25 @@ template <typename R> int ens_tune(Ensemble& e, std::string& err) {
           bord = ord;
         }
       }
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
-  off = 0;
-  for (auto& mb : e.m) {
-    void* f[3] = {mb.b1, mb.b2, mb.b3};
-    for (int j = 0; j < 3; ++j) {
-      (void)hipMemcpyAsync(f[j], save + off, member_bytes(e, mb), hipMemcpyDeviceToDevice, e.stream);
-      off += member_bytes(e, mb);
-    }
-  }
-  (void)hipStreamSynchronize(e.stream);
   int rc = ens_configure<R>(e, bty, bzc, bord, err);
   if (rc) return rc;
   rc = ens_capture<R>(e, err);
   if (rc) return rc;
-  {  // graph replay against direct launches of the same period (fields restored afterwards)
-    hipEvent_t u0, u1, u2;
-    (void)hipEventCreate(&u0);
-    (void)hipEventCreate(&u1);
-    (void)hipEventCreate(&u2);
-    off = 0;
-    (void)hipGraphLaunch(e.exec, e.stream);  // warm-up
-    (void)hipEventRecord(u0, e.stream);
-    for (int k = 0; k < 2; ++k) (void)hipGraphLaunch(e.exec, e.stream);
-    (void)hipEventRecord(u1, e.stream);
-    for (int k = 0; k < 2; ++k) (void)ens_launch_steps<R>(e, e.stream);
-    (void)hipEventRecord(u2, e.stream);
-    (void)hipEventSynchronize(u2);
-    float tg = 0.f, td = 0.f;
-    (void)hipEventElapsedTime(&tg, u0, u1);
-    (void)hipEventElapsedTime(&td, u1, u2);
-    e.use_graph = tg <= td;
-    (void)hipEventDestroy(u0);
-    (void)hipEventDestroy(u1);
-    (void)hipEventDestroy(u2);
-    for (auto& mb : e.m) {
-      void* f[3] = {mb.b1, mb.b2, mb.b3};
-      for (int j = 0; j < 3; ++j) {
-        (void)hipMemcpyAsync(f[j], save + off, member_bytes(e, mb), hipMemcpyDeviceToDevice, e.stream);
-        off += member_bytes(e, mb);
-      }
-    }
-    (void)hipStreamSynchronize(e.stream);
-  }
-  (void)hipFree(save);
+  // graph replay against direct launches of the same period (fields restored by the scope)
+  (void)hipGraphLaunch(e.exec, e.stream);  // warm-up
+  (void)hipEventRecord(t0, e.stream);
+  for (int k = 0; k < 2; ++k) (void)hipGraphLaunch(e.exec, e.stream);
+  (void)hipEventRecord(t1, e.stream);
+  for (int k = 0; k < 2; ++k) (void)ens_launch_steps<R>(e, e.stream);
+  (void)hipEventRecord(t2, e.stream);
+  (void)hipEventSynchronize(t2);
+  float tg = 0.f, td = 0.f;
+  (void)hipEventElapsedTime(&tg, t0, t1);
+  (void)hipEventElapsedTime(&td, t1, t2);
+  e.use_graph = tg <= td;
+  if (hipGetLastError() != hipSuccess) return err = "ensemble tuning failed", -4;
   e.tuned = true;
-  return hipGetLastError() == hipSuccess ? 0 : (err = "ensemble tuning failed", -4);
+  return 0;
 }
 
 }  // namespace
@@ -407,10 +403,16 @@ int ensemble_step(Ensemble* e, long long nsteps, double* seconds, std::string& e
     if (const char* g = getenv("IEQ_ENS_GRAPH")) e->use_graph = atoi(g) != 0;
   }
   if (nsteps % e->steps_per_graph) return err = "this integrator advances the ensemble in pairs of steps", IEQ_ERR_ARGS;
-  hipEvent_t t0, t1;
-  (void)hipEventCreate(&t0);
-  (void)hipEventCreate(&t1);
-  (void)hipEventRecord(t0, e->stream);
+  hipEvent_t t[2] = {nullptr, nullptr};
+  struct Ev {
+    hipEvent_t* t;
+    ~Ev() {
+      for (int i = 0; i < 2; ++i)
+        if (t[i]) (void)hipEventDestroy(t[i]);
+    }
+  } guard{t};
+  if (hipEventCreate(&t[0]) != hipSuccess || hipEventCreate(&t[1]) != hipSuccess) return err = "event creation failed", IEQ_ERR_HIP;
+  (void)hipEventRecord(t[0], e->stream);
   for (long long k = 0; k < nsteps; k += e->steps_per_graph) {
     if (e->use_graph) {
       if (hipGraphLaunch(e->exec, e->stream) != hipSuccess) return err = "graph launch failed", IEQ_ERR_HIP;
@@ -419,12 +421,10 @@ int ensemble_step(Ensemble* e, long long nsteps, double* seconds, std::string& e
       if (r != hipSuccess) return err = std::string("ensemble launch failed: ") + hipGetErrorString(r), IEQ_ERR_HIP;
     }
   }
-  (void)hipEventRecord(t1, e->stream);
-  if (hipEventSynchronize(t1) != hipSuccess) return err = "ensemble step failed", IEQ_ERR_HIP;
+  (void)hipEventRecord(t[1], e->stream);
+  if (hipEventSynchronize(t[1]) != hipSuccess) return err = "ensemble step failed", IEQ_ERR_HIP;
   float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, t0, t1);
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
+  (void)hipEventElapsedTime(&ms, t[0], t[1]);
   if (seconds) *seconds = ms * 1e-3;
   return hipGetLastError() == hipSuccess ? IEQ_OK : (err = "ensemble kernel failed", IEQ_ERR_HIP);
 }

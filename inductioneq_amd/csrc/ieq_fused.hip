@@ -9,9 +9,11 @@
 //               products J_a B_b - J_b B_a, regrouped per component of E
 //   dissipation (artificial_dissipation.cl:13-32): F * sum_d sum_m w_m B(x + m e_d),
 //               w = sign * (central D_HO row convolved with itself)
-// Two kernels per stage: k_emf writes E; k_stage reads B and E and applies the RK
-// update.  Both march along z with register queues of the z-stencil and stage one
-// xy-plane (plus halo) at a time in LDS for the x/y stencils.
+// With the Hall term a stage is one kernel, k_hall_stage (ieq_hall_tiles.hpp), which forms E
+// on its tile and halo itself; IEQ_FUSED_SPLIT=1 selects the two-kernel form instead (k_emf
+// writes E, k_stage reads B and E).  Without the Hall term a stage is one k_stage.  All
+// kernels march along z with register queues of the z-stencil and stage one xy-plane (plus
+// halo) at a time in LDS for the x/y stencils.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -22,6 +24,7 @@
 #include "ieq_fused.hpp"
 #include "ieq_fused_tiles.hpp"
 #include "ieq_generic.hpp"
+#include "ieq_hall.hpp"
 #include "ieq_stencil.hpp"
 
 namespace ieq {
@@ -116,8 +119,13 @@ template <typename R> __global__ void k_uniform_check(const R* u, const R* rho, 
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     double v[4] = {(double)u[i], (double)u[S + i], (double)u[2 * S + i], (double)rho[i]};
     for (int c = 0; c < 4; ++c) {
-      mn[c] = fmin(mn[c], v[c]);
-      mx[c] = fmax(mx[c], v[c]);
+      if (v[c] != v[c]) {  // NaN: never uniform (fmin/fmax would skip it)
+        mn[c] = -INFINITY;
+        mx[c] = INFINITY;
+      } else {
+        mn[c] = fmin(mn[c], v[c]);
+        mx[c] = fmax(mx[c], v[c]);
+      }
     }
   }
   __shared__ double smn[4][256], smx[4][256];
@@ -417,7 +425,8 @@ namespace {
 
 template <typename R>
 int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, FusedWork** work, const char* integrator,
-                 long long nsteps, void* const* args, int* used, std::string& err) {
+                 long long nsteps, void* const* args, const FusedOpts& opt, int* used, std::string& err) {
+  const double* uniform = opt.uniform;
   std::vector<StageSpec> plan;
   int nf = 0;
   if (!fused_plan(integrator, p.hall, plan, nf)) return IEQ_OK;
@@ -434,26 +443,37 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
 
   if (!*work) *work = new FusedWork();
   FusedWork* w = *work;
-  const int nblk = 256;
-  if (!w->red) {
-    if (hipMalloc(&w->red, nblk * 8 * sizeof(double)) != hipSuccess) return err = "hipMalloc failed", IEQ_ERR_HIP;
-    if (hipHostMalloc(&w->red_host, nblk * 8 * sizeof(double)) != hipSuccess) return err = "hipHostMalloc failed", IEQ_ERR_HIP;
-  }
-  k_uniform_check<R><<<nblk, 256, 0, s>>>(u, rho, info.num_nodes, info.stride, w->red);
-  if (hipMemcpyAsync(w->red_host, w->red, nblk * 8 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess)
-    return err = "uniformity check failed", IEQ_ERR_HIP;
   double mn[4], mx[4];
-  for (int c = 0; c < 4; ++c) {
-    mn[c] = 1e300;
-    mx[c] = -1e300;
-    for (int b = 0; b < nblk; ++b) {
-      mn[c] = std::fmin(mn[c], w->red_host[b * 8 + c]);
-      mx[c] = std::fmax(mx[c], w->red_host[b * 8 + 4 + c]);
+  if (uniform) {  // the caller states the uniform values: no device check, no host synchronisation
+    for (int c = 0; c < 4; ++c) mn[c] = mx[c] = uniform[c];
+  } else {
+    const int nblk = 256;
+    if (!w->red || !w->red_host) {
+      if (w->red) (void)hipFree(w->red);
+      w->red = nullptr;
+      if (hipMalloc(&w->red, nblk * 8 * sizeof(double)) != hipSuccess) return w->red = nullptr, err = "hipMalloc failed", IEQ_ERR_HIP;
+      if (hipHostMalloc(&w->red_host, nblk * 8 * sizeof(double)) != hipSuccess) {
+        (void)hipFree(w->red);
+        w->red = nullptr;
+        w->red_host = nullptr;
+        return err = "hipHostMalloc failed", IEQ_ERR_HIP;
+      }
+    }
+    k_uniform_check<R><<<nblk, 256, 0, s>>>(u, rho, info.num_nodes, info.stride, w->red);
+    if (hipMemcpyAsync(w->red_host, w->red, nblk * 8 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return err = "uniformity check failed", IEQ_ERR_HIP;
+    for (int c = 0; c < 4; ++c) {
+      mn[c] = 1e300;
+      mx[c] = -1e300;
+      for (int b = 0; b < nblk; ++b) {
+        mn[c] = std::fmin(mn[c], w->red_host[b * 8 + c]);
+        mx[c] = std::fmax(mx[c], w->red_host[b * 8 + 4 + c]);
+      }
     }
   }
   for (int c = 0; c < 4; ++c)
-    if (mn[c] != mx[c]) return IEQ_OK;  // not uniform: reference kernel sequence
+    if (!(mn[c] == mx[c])) return IEQ_OK;  // not uniform (or NaN): reference kernel sequence
   if (!(mn[3] != 0.0)) return IEQ_OK;
 
   const size_t vbytes = (size_t)3 * info.stride * sizeof(R);
@@ -515,12 +535,31 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
     }
     a.zc = pick_zc<R>(rr, ty, t, p.hall, hod);
   }
-  int zc_emf = env_int("IEQ_EMF_ZC", 0);
-  if (zc_emf < 1) zc_emf = pick_zc_emf<R>(rr, ty, a);
+  // One kernel per Hall stage unless split on request or the 32-bit byte offsets of the staged
+  // halo (which include the component) would not fit.
+  const bool one_kernel = p.hall && env_int("IEQ_FUSED_SPLIT", 0) == 0 && (long long)3 * info.stride * sizeof(R) < (1LL << 32);
+  int zc_emf = 0, zc_hall = 0;
+  if (p.hall && !one_kernel) {
+    zc_emf = env_int("IEQ_EMF_ZC", 0);
+    if (zc_emf < 1) zc_emf = pick_zc_emf<R>(rr, ty, a);
+  }
+  if (one_kernel) {
+    zc_hall = env_int("IEQ_FUSED_ZC", 0);
+    if (zc_hall < 1) {
+      FArgs<R> t = a;
+      t.zc = 1;
+      t.mode = plan.back().kind == 4 ? 1 : 0;
+      t.Y = b1;
+      t.b2 = F[1];
+      zc_hall = pick_zc_hall<R>(rr, t, hod);
+    }
+  }
+  const R* y_last = nullptr;  // field differentiated by the last stage (curl B / rho of it on request)
   for (long long step = 0; step < nsteps; ++step) {
     for (const StageSpec& st : plan) {
       a.P = nullptr;
       a.Q = nullptr;
+      if (st.kind == 3 && one_kernel) continue;  // the stage kernel forms E itself
       if (st.kind == 3) {  // EMF of the field the next stage differentiates
         a.Y = st.Y == 0 ? b1 : F[st.Y];
         a.Eout = J;
@@ -538,7 +577,14 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
         a.cP = (R)st.cP;
         a.cQ = (R)st.cQ;
         a.cD = (R)(st.cD * p.dt);
-        stage<R>(rr, ty, a, p.hall, hod, s);
+        y_last = a.Y;
+        if (one_kernel) {
+          FArgs<R> ah = a;
+          ah.zc = zc_hall;
+          launch_hall_stage<R>(rr, ah, hod, s);
+        } else {
+          stage<R>(rr, ty, a, p.hall, hod, s);
+        }
       } else if (st.kind == 1) {
         uint32_t g[3] = {(uint32_t)p.nx, (uint32_t)p.ny, (uint32_t)p.nz};
         R* FF[3] = {b1, F[1], F[2]};
@@ -555,7 +601,14 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
         a.cP = (R)st.cP;
         a.cQ = (R)st.cQ;
         a.cD = DT;
-        stage<R>(rr, ty, a, p.hall, hod, s);
+        y_last = a.Y;
+        if (one_kernel) {
+          FArgs<R> ah = a;
+          ah.zc = zc_hall;
+          launch_hall_stage<R>(rr, ah, hod, s);
+        } else {
+          stage<R>(rr, ty, a, p.hall, hod, s);
+        }
         R* t = b1;
         b1 = b1alt;
         b1alt = t;
@@ -563,6 +616,12 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return err = std::string("fused launch: ") + hipGetErrorString(e), IEQ_ERR_HIP;
     }
+  }
+  if (opt.final_curl && p.hall && y_last) {
+    // curlB_rho as the reference schedule leaves it: calc_curlB_rho of the last stage's input
+    // (still intact: a register of the 3-register schemes, the ping-pong copy of the 2N ones)
+    const uint32_t g[3] = {(uint32_t)p.nx, (uint32_t)p.ny, (uint32_t)p.nz};
+    gen_curl_rho<R>(p, info.opset, s, g, y_last, J, rho);
   }
   if (b1 != F[0]) {
     if (hipMemcpyAsync(F[0], b1, vbytes, hipMemcpyDeviceToDevice, s) != hipSuccess) return err = "copy-back failed", IEQ_ERR_HIP;
@@ -576,11 +635,11 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
 
 
 int fused_step(const Params& p, const ieq_program_info& info, hipStream_t s, FusedWork** work, const char* integrator,
-               long long nsteps, int variable_u, void* const* args, int* used, std::string& err) {
+               long long nsteps, int variable_u, void* const* args, const FusedOpts& opt, int* used, std::string& err) {
   *used = 0;
   if (!p.periodic || variable_u || (p.diss != 0 && p.diss != 1)) return IEQ_OK;
-  if (info.real_bytes == 8) return fused_step_t<double>(p, info, s, work, integrator, nsteps, args, used, err);
-  return fused_step_t<float>(p, info, s, work, integrator, nsteps, args, used, err);
+  if (info.real_bytes == 8) return fused_step_t<double>(p, info, s, work, integrator, nsteps, args, opt, used, err);
+  return fused_step_t<float>(p, info, s, work, integrator, nsteps, args, opt, used, err);
 }
 
 }  // namespace ieq

@@ -17,11 +17,20 @@ struct FusedWork;
 // Returns 0, or -1 for an unknown integrator.
 int schedule(const char* integrator, int hall, int variable_u, std::vector<std::string>& names, int& nf);
 
+struct FusedOpts {
+  // (u_x, u_y, u_z, rho) of the uniform coefficient fields as stated by the caller; null: the
+  // fields are checked on the device (one host synchronisation)
+  const double* uniform = nullptr;
+  // leave curlB_rho = curl B / rho of the last stage's input, as the reference schedule does
+  // (otherwise it holds a work field)
+  bool final_curl = false;
+};
+
 // Advance nsteps with the fused kernels when the program qualifies (periodic grid,
 // uniform u and rho, no time-dependent velocity, no dissipation other than high
 // order).  Sets *used = 0 and enqueues nothing when it does not qualify.
 int fused_step(const Params& p, const ieq_program_info& info, hipStream_t s, FusedWork** work, const char* integrator,
-               long long nsteps, int variable_u, void* const* args, int* used, std::string& err);
+               long long nsteps, int variable_u, void* const* args, const FusedOpts& opt, int* used, std::string& err);
 
 void fused_release(FusedWork* w);
 

@@ -1,0 +1,19 @@
+// Single-kernel RK stage of the periodic Hall problem (EMF formed in the stage kernel).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ieq_stencil.hpp"
+
+namespace ieq {
+
+// Stage kernel for the arguments (update form from a.mode / P / Q / Y, convection when a.conv),
+// single-grid or batched over a tile list (int4 {x0, y0, z0, member}, FArgs per member).
+template <typename R> const void* hall_stage_kernel(int rr, const FArgs<R>& a, bool hod, bool batched);
+// Tile shape (threads per workgroup = tx * ty) of the kernels above.
+template <typename R> void hall_tile_shape(int* tx, int* ty);
+template <typename R> void launch_hall_stage(int rr, const FArgs<R>& a, bool hod, hipStream_t s);
+// z-chunk that makes the tile grid one round of resident workgroups.
+template <typename R> int pick_zc_hall(int rr, const FArgs<R>& a, bool hod);
+
+}  // namespace ieq

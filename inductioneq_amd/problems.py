@@ -9,6 +9,9 @@ ix + NX*(iy + NY*iz).
   scaled by ``1/|k|^2``; seeded with ``numpy.random.default_rng(seed)``.
 * :func:`gradient_potential` -- the gradient of a random scalar potential
   ``psi = sum_{|k|<=kmax} c_k cos(pi k.(x+1)/2)`` on ``[-1, 1]^3`` (injects divergence).
+* :func:`beltrami_hall_field` -- an exact periodic solution of the implemented equation
+  with uniform velocity and density: B = B0 + eps*b(x - v t), b the ABC field of
+  wavenumber m (curl b = m b), v = u + (m/rho) B0.
 * :func:`hall_dt` -- the reference's CFL rule ``dt = cfl*min(dx)/max|u|``
   (``matlab/+induction_eq/initialize.m:93-96``) is undefined for ``u = 0``; for the
   Hall term the step is set from its spectral radius instead:
@@ -65,6 +68,27 @@ def random_modes_field(n: Sequence[int], seed: int, kmax: int = 4, amp: float = 
         c, s = ph.real, ph.imag
         B += c[..., None] * a + s[..., None] * b
     return B.reshape(-1, 3)
+
+
+def beltrami_hall_field(n: Sequence[int], t: float = 0.0, m: int = 1, abc: Tuple[float, float, float] = (0.3, 0.2, 0.25),
+                        eps: float = 1.0, background: Tuple[float, float, float] = (0.3, -0.2, 1.0),
+                        u: Tuple[float, float, float] = (0.0, 0.0, 0.0), rho: float = 1.0,
+                        length: float = 2 * math.pi):
+    """Exact solution of dB/dt = curl(u x B) - curl((curl B)/rho x B) on the periodic cube for uniform u and rho.
+
+    b = (A sin(m z) + C cos(m y), B sin(m x) + A cos(m z), C sin(m y) + B cos(m x)) satisfies curl b = m b
+    and div b = 0, so with B = B0 + eps*b: J x B = (m/rho) eps b x B0, and both terms reduce to transport,
+    dB/dt = -((u + (m/rho) B0) . grad) B, whose solution is B(x, t) = B0 + eps*b(x - v t), v = u + (m/rho) B0."""
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    v = np.asarray(u, float) + (m / rho) * np.asarray(background, float)
+    x = np.arange(nx) * (length / nx) - v[0] * t
+    y = np.arange(ny) * (length / ny) - v[1] * t
+    z = np.arange(nz) * (length / nz) - v[2] * t
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    A, Bc, C = abc
+    b = np.stack([A * np.sin(m * Z) + C * np.cos(m * Y), Bc * np.sin(m * X) + A * np.cos(m * Z),
+                  C * np.sin(m * Y) + Bc * np.cos(m * X)], axis=-1)
+    return (np.asarray(background, float) + eps * b).reshape(-1, 3)
 
 
 def gradient_potential(n: Sequence[int], seed: int, kmax: int = 6, lo: float = -1.0, hi: float = 1.0):

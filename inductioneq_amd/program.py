@@ -164,13 +164,26 @@ class Program:
                     self.handle)
         return sec.value
 
-    def step(self, integrator: str, nsteps: int, args, variable_u: bool = False, timed: bool = False):
-        """Advance nsteps full time steps (fused kernels where the program qualifies)."""
+    def step(self, integrator: str, nsteps: int, args, variable_u: bool = False, timed: bool = False, uniform=None):
+        """Advance nsteps full time steps (fused kernels where the program qualifies).
+
+        ``uniform = (u_x, u_y, u_z, rho)`` states that the u and rho fields hold these values at every
+        node (ieq_step_uniform: no device check, no host synchronisation)."""
         sec = ctypes.c_double(0.0)
-        rc = self.lib.ieq_step(self.handle, integrator.encode(), int(nsteps), int(variable_u), self._ptrs(args), len(args),
-                               ctypes.byref(sec) if timed else None)
+        if uniform is not None:
+            u3 = (ctypes.c_double * 3)(*[float(v) for v in uniform[:3]])
+            rc = self.lib.ieq_step_uniform(self.handle, integrator.encode(), int(nsteps), u3, float(uniform[3]),
+                                           self._ptrs(args), len(args), ctypes.byref(sec) if timed else None)
+        else:
+            rc = self.lib.ieq_step(self.handle, integrator.encode(), int(nsteps), int(variable_u), self._ptrs(args),
+                                   len(args), ctypes.byref(sec) if timed else None)
         _capi.check(rc, self.handle)
         return sec.value if timed else None
+
+    @property
+    def last_step_fused(self) -> bool:
+        """True when the last run / step call ran the fused stage kernels (ieq_last_step_path)."""
+        return self.lib.ieq_last_step_path(self.handle) == 1
 
     def reduce(self, name: str, *fields) -> float:
         """Run a group-reduction kernel and return the host sum of its partials."""

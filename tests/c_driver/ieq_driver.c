@@ -5,9 +5,12 @@
  *
  * Builds the program from the generate_settings string and module list, uploads the
  * AoS REAL4 field of in.bin (num_nodes_pad x 4 REALs), runs nsteps steps of the
- * integrator three times: through ieq_step, through ieq_run with the reference kernel
- * list, and through the typed per-kernel entry points (inductioneq_kernels.h) with the
- * same list; writes the three results (AoS REAL4, in that order) to out.bin. */
+ * integrator four times: through ieq_step, through ieq_run with the reference kernel
+ * list once per step, through the typed per-kernel entry points (inductioneq_kernels.h)
+ * with the same list, and through one ieq_run on the list repeated nsteps times (the
+ * reference's repmat(RK_Step, 1, nsteps), compute_numerical_solution.m:91-108); writes the
+ * four results (AoS REAL4, in that order) to out.bin and prints the path each one took
+ * (ieq_last_step_path: 1 fused kernels, 0 reference kernels). */
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -61,14 +64,16 @@ int main(int argc, char **argv) {
   ieq_program_info info;
   if (ieq_program_get_info(p, &info) != IEQ_OK) return fail(p, "info");
   const size_t n = (size_t)info.num_nodes_pad, rb = (size_t)info.real_bytes;
-  void *host = malloc(n * 4 * rb), *out = malloc(3 * n * 4 * rb);
+  const int npaths = 4;
+  void *host = malloc(n * 4 * rb), *out = malloc(npaths * n * 4 * rb);
   FILE *f = fopen(argv[sep + 3], "rb");
   if (!f || fread(host, rb, n * 4, f) != n * 4) {
     fprintf(stderr, "ieq_driver: cannot read %s\n", argv[sep + 3]);
     return 1;
   }
   fclose(f);
-  for (int path = 0; path < 3; ++path) {
+  printf("ieq_driver: paths");
+  for (int path = 0; path < npaths; ++path) {
     /* b1, b2, curlB_rho, u, rho, time (kernel_time_integrator.cl:1023-1026) */
     void *b1, *b2, *J, *u, *rho, *t;
     const size_t vb = ieq_vector_field_bytes(p), sb = ieq_scalar_field_bytes(p);
@@ -85,25 +90,36 @@ int main(int argc, char **argv) {
     if (ieq_upload_real(p, rho, ones, n) != IEQ_OK) return fail(p, "upload rho");
     free(ones);
     void *args[6] = {b1, b2, J, u, rho, t};
+    const int nk = (int)(sizeof(ck54) / sizeof(ck54[0]));
     if (path == 0) {
       if (ieq_step(p, integrator, nsteps, 0, args, 6, NULL) != IEQ_OK) return fail(p, "ieq_step");
+    } else if (path == 3) {
+      const uint32_t g[3] = {(uint32_t)info.nx, (uint32_t)info.ny, (uint32_t)info.nz}, l[3] = {(uint32_t)info.w_size, 1, 1};
+      const char **list = (const char **)malloc((size_t)nsteps * nk * sizeof(char *));
+      for (long long s = 0; s < nsteps; ++s)
+        for (int k = 0; k < nk; ++k) list[s * nk + k] = ck54[k];
+      double sec = 0;
+      if (ieq_run(p, list, (int)(nsteps * nk), g, l, args, 6, &sec) != IEQ_OK) return fail(p, "ieq_run (repeated list)");
+      free(list);
     } else {
       const uint32_t g[3] = {(uint32_t)info.nx, (uint32_t)info.ny, (uint32_t)info.nz}, l[3] = {(uint32_t)info.w_size, 1, 1};
       for (long long s = 0; s < nsteps; ++s) {
         if (path == 1) {
-          if (ieq_run(p, ck54, (int)(sizeof(ck54) / sizeof(ck54[0])), g, l, args, 6, NULL) != IEQ_OK) return fail(p, "ieq_run");
+          if (ieq_run(p, ck54, nk, g, l, args, 6, NULL) != IEQ_OK) return fail(p, "ieq_run");
         } else {
           for (size_t k = 0; k < sizeof(ck54_typed) / sizeof(ck54_typed[0]); ++k)
             if (ck54_typed[k](p, g, l, b1, b2, J, u, rho, t, NULL) != IEQ_OK) return fail(p, ck54[k]);
         }
       }
     }
+    printf(" %d", ieq_last_step_path(p));
     if (ieq_download_real4(p, (char *)out + path * n * 4 * rb, b1, n) != IEQ_OK) return fail(p, "download");
     void *all[6] = {b1, b2, J, u, rho, t};
     for (int k = 0; k < 6; ++k) ieq_free(p, all[k]);
   }
+  printf("\n");
   f = fopen(argv[sep + 4], "wb");
-  if (!f || fwrite(out, rb, 3 * n * 4, f) != 3 * n * 4) {
+  if (!f || fwrite(out, rb, npaths * n * 4, f) != npaths * n * 4) {
     fprintf(stderr, "ieq_driver: cannot write %s\n", argv[sep + 4]);
     return 1;
   }

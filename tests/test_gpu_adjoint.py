@@ -209,3 +209,67 @@ def test_autograd_function_matches_solver():
     assert torch.equal(b.grad, g)
     assert float(c.grad) == gc and [float(v) for v in u.grad] == gu
     assert abs(float(loss) - _loss(cfg, solver.forward(b0, u0, ch0), tgt)) <= 1e-12 * float(loss)
+
+
+def _cfg4(real, nsteps, snapshots):
+    """The gradient workload at its full size: 96^3 periodic, 6th order + HO dissipation, SSPRK33, u = (0.2, -0.1,
+    0.15), c_H = 1, B0 = (0,0,1) + random modes (seed 1), target from seed 2."""
+    import torch
+
+    from inductioneq_amd.adjoint import AdjointSolver
+    from inductioneq_amd.problems import hall_dt, random_modes_field
+    from inductioneq_amd.program import Program
+
+    n, L = 96, 2 * math.pi
+    B0, Bt = random_modes_field(n, seed=1), random_modes_field(n, seed=2)
+    h = L / n
+    cfg = ProgramConfig(real=real, opset="6thOrder", nx=n, ny=n, nz=n, xmin=0, xmax=L, ymin=0, ymax=L, zmin=0, zmax=L,
+                        periodic=True, hall=True, dissipation="HIGH_ORDER", testcase="uniform_flow", w_size=256,
+                        dt=hall_dt(B0, (h, h, h), 6, c_hall=1.0, cfl=0.4))
+    prog = Program(cfg)
+    solver = AdjointSolver(prog, nsteps, snapshots)
+    b0, tgt = prog.vector(), prog.vector()
+    b0[:, : cfg.num_nodes] = torch.as_tensor(B0.T, dtype=prog.torch_dtype)
+    tgt[:, : cfg.num_nodes] = torch.as_tensor(Bt.T, dtype=prog.torch_dtype)
+    return cfg, prog, solver, b0, tgt
+
+
+def test_dot_product_test_at_96():
+    """<J v, w> = <v, J^T w> at the gradient workload's size (FP64, 5 steps, J v by central differences)."""
+    import torch
+
+    cfg, prog, solver, b0, _ = _cfg4("double", 5, 2)
+    u, ch = [0.2, -0.1, 0.15], 1.0
+    n = cfg.num_nodes
+    rng = np.random.default_rng(31)
+    v, w = prog.vector(), prog.vector()
+    v[:, :n] = torch.as_tensor(rng.standard_normal((3, n)), dtype=torch.float64)
+    w[:, :n] = torch.as_tensor(rng.standard_normal((3, n)), dtype=torch.float64)
+    eps = 1e-5
+    Jv = (solver.forward(b0 + eps * v, u, ch) - solver.forward(b0 - eps * v, u, ch)) / (2 * eps)
+    solver.forward(b0, u, ch)
+    JTw, _, _ = solver.backward(w)
+    lhs = float((Jv[:, :n] * w[:, :n]).sum())
+    rhs = float((v[:, :n] * JTw[:, :n]).sum())
+    print("dot-product test 96^3: %.15e vs %.15e" % (lhs, rhs))
+    assert abs(lhs - rhs) <= 1e-8 * abs(lhs), (lhs, rhs)
+
+
+def test_binomial_15_snapshots_equals_store_all_at_96():
+    """200 steps at 96^3 (FP32): the gradient with 15 binomial snapshots is bitwise the one with every stage
+    input stored, and the recomputation count is the binomial schedule's."""
+    import torch
+
+    u, ch = [0.2, -0.1, 0.15], 1.0
+    out = []
+    for snaps in (15, 200):
+        cfg, prog, solver, b0, tgt = _cfg4("float", 200, snaps)
+        g, gc, gu = _grad(cfg, solver, b0, tgt, u, ch)
+        torch.cuda.synchronize()
+        out.append((g.cpu(), gc, tuple(gu), solver.counts()))
+        del solver, prog
+    (g15, gc15, gu15, c15), (ga, gca, gua, ca) = out
+    assert torch.equal(g15, ga) and gc15 == gca and gu15 == gua
+    assert ca == (200, 200)
+    assert c15[1] == 200 and 200 < c15[0] <= 3 * 200
+    assert float(g15.abs().max()) > 0

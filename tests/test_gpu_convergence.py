@@ -4,7 +4,9 @@ The reference's convergence studies (paper/convergence_hall_periodic.m:104-152,
 matlab/compute_error_rotation_2D.m) run a testcase to a final time and measure the relative
 M-norm error against `b_analytical`.  The same flow runs here on the GPU (harness.initialize
 and compute_numerical_solution over the HIP library): hall_periodic (nonlinear Hall term,
-time-dependent velocity, periodic; include/hall_periodic.h) through ieq_step, and
+time-dependent velocity, periodic; include/hall_periodic.h) through ieq_step, which runs the
+reference kernel sequence for a time-dependent velocity (the fused kernels need uniform u; their
+convergence is tested against an exact solution in test_gpu_fused.py), and
 rotation_2D (inflow SATs with analytic boundary data; include/rotation_2D.h) through ieq_run
 kernel lists.  Each error must equal the CPU oracle's error for the same run (to 1e-10 of the
 analytic field's norm), and halving
@@ -30,8 +32,11 @@ def _errors(ops, n, integrator, tc, hall, variable_u, T_end, cfl, periodic, lo, 
                             periodic=periodic, hall=hall, testcase=tc, w_size=16)
         init = initialize(cfg, make, cfl, T_end)
         cfg.dt = init["dt"]
-        res = compute_numerical_solution(cfg, make(cfg), integrator, init["b"], init["u"], init["rho"], init["num_steps"],
+        ex = make(cfg)
+        res = compute_numerical_solution(cfg, ex, integrator, init["b"], init["u"], init["rho"], init["num_steps"],
                                          T_end, variable_u=variable_u, fused=fused and make is HipExecutor)
+        if make is HipExecutor and fused and variable_u:
+            assert not ex.prog.last_step_fused  # calc_u before every stage: reference kernels
         out.append(res["rel_err"])
     return out  # (HIP, oracle)
 

@@ -1,0 +1,299 @@
+"""The fused stage kernels at their production launch shapes, against the CPU oracle.
+
+* cfg1 at its full size: one CarpenterKennedy2N54 step of the periodic 128^3 Hall problem
+  (6th order + HO dissipation, u = 0, rho = 1, B0 = (0,0,1) + random modes |k| <= 4, seed 1)
+  through ieq_step, which picks the production tile shape and occupancy-sized z-chunks;
+  FP64 against the FP64 oracle to 1e-12, FP32 against the FP32 oracle to 2e-5.
+* forced z-chunks (IEQ_FUSED_ZC / IEQ_EMF_ZC = 2, 3, 4) so every tile runs the multi-chunk
+  lead-in and a partial last chunk.
+* the exact solution B0 + eps*b(x - (u + m B0/rho) t) of the implemented equation for an ABC
+  field b (curl b = m b) with uniform u and rho: the fused path converges to it at the
+  operator's order.
+* ieq_run on a repeated RK_Step list (the reference's repmat(RK_Step, 1, k)) takes the fused
+  kernels and leaves b1, the other registers and time as the reference kernel sequence does;
+  ieq_run_host does the same on host arrays for a 1000-step list.
+* ieq_step_uniform (values stated by the caller) equals ieq_step bitwise; lists or fields that
+  do not qualify (time-dependent velocity, non-uniform or NaN coefficients, a partial NDRange)
+  run the reference kernels.
+Every test that expects the fused kernels asserts that they ran (ieq_last_step_path).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from inductioneq_amd.problems import beltrami_hall_field, hall_dt, random_modes_field, to_aos4
+from inductioneq_amd.schedules import INTEGRATORS, rk_schedule
+from inductioneq_amd.settings import ProgramConfig
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"double": 1e-12, "float": 2e-5}
+L2PI = 2 * math.pi
+
+
+def _periodic_cfg(n, real="double", ops="6thOrder", hall=True, diss="HIGH_ORDER", u=(0.0, 0.0, 0.0), **kw):
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    kw.setdefault("w_size", 64)
+    return ProgramConfig(real=real, opset=ops, nx=nx, ny=ny, nz=nz, xmin=0, xmax=L2PI, ymin=0, ymax=L2PI, zmin=0,
+                         zmax=L2PI, periodic=True, hall=hall, dissipation=diss, testcase="uniform_flow", ux=u[0],
+                         uy=u[1], uz=u[2], **kw)
+
+
+def _relerr(got, want):
+    return float(np.abs(got - want).max() / max(np.abs(want).max(), 1e-300))
+
+
+class _Env:
+    """Temporarily set environment variables read by the library at each call."""
+
+    def __init__(self, **kv):
+        self.kv = {k: str(v) for k, v in kv.items()}
+        self.old = {}
+
+    def __enter__(self):
+        for k, v in self.kv.items():
+            self.old[k] = os.environ.get(k)
+            os.environ[k] = v
+
+    def __exit__(self, *exc):
+        for k, v in self.old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def _device_args(prog, B, nf, u, rho):
+    import torch
+
+    n = prog.config.num_nodes
+    b = [prog.vector() for _ in range(nf)]
+    b[0][:, :n] = torch.as_tensor(B.T, dtype=prog.torch_dtype)
+    uf = prog.vector()
+    for k in range(3):
+        uf[k, :n] = u[k]
+    return b + [prog.vector(), uf, prog.scalar(rho), prog.time_buffer()]
+
+
+def _oracle_steps(cfg, B, integrator, nsteps, u, rho):
+    from pyoracle import Oracle
+
+    orc = Oracle(cfg)
+    names, nf = rk_schedule(integrator, cfg.hall, False)
+    hb = [to_aos4(B, cfg.num_nodes_pad, orc.dtype)] + [orc.vec() for _ in range(nf - 1)]
+    hu = to_aos4(np.tile(np.asarray(u, float), (cfg.num_nodes, 1)), cfg.num_nodes_pad, orc.dtype)
+    ht = orc.time()
+    orc.run(names * nsteps, hb + [orc.vec(), hu, orc.scal(rho), ht])
+    return hb, ht
+
+
+@pytest.mark.parametrize("real", ["double", "float"])
+def test_hall128_step_matches_oracle(real):
+    """cfg1 at 128^3: one CK2N54 step through ieq_step (production launch shape) vs the oracle."""
+    import torch
+
+    from inductioneq_amd.program import Program
+
+    n = 128
+    cfg = _periodic_cfg(n, real, w_size=256)
+    B0 = random_modes_field(n, seed=1)
+    cfg.dt = hall_dt(B0, (cfg.dx, cfg.dy, cfg.dz), 6, c_hall=1.0, cfl=0.5)
+    prog = Program(cfg)
+    args = _device_args(prog, B0, 2, (0.0, 0.0, 0.0), 1.0)
+    prog.step("CarpenterKennedy2N54", 1, args)
+    torch.cuda.synchronize()
+    assert prog.last_step_fused
+    hb, ht = _oracle_steps(cfg, B0, "CarpenterKennedy2N54", 1, (0.0, 0.0, 0.0), 1.0)
+    got = args[0][:, : cfg.num_nodes].double().cpu().numpy().T
+    want = hb[0][: cfg.num_nodes, :3].astype(np.float64)
+    e = _relerr(got, want)
+    print("hall128 %s one CK2N54 step: rel err vs oracle %.3e" % (real, e))
+    assert e <= TOL[real], e
+    # the state moved: the step is not the identity at this tolerance
+    assert _relerr(want, B0) > 100 * TOL[real]
+    assert abs(float(args[-1][0]) - float(ht[0])) <= (1e-15 if real == "double" else 1e-7)
+
+
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("real", ["double", "float"])
+@pytest.mark.parametrize("zc", [2, 3, 4])
+@pytest.mark.parametrize("integrator", ["CarpenterKennedy2N54", "SSPRK33"])
+def test_forced_z_chunks_match_oracle(real, zc, integrator, split):
+    """Every tile in a multi-chunk z-march (chunk lead-in planes, partial last chunk) vs the oracle, for the
+    one-kernel Hall stage and the two-kernel form (IEQ_FUSED_SPLIT=1: EMF kernel + stage kernel)."""
+    import torch
+
+    from inductioneq_amd.program import Program
+
+    n = (20, 24, 22)
+    cfg = _periodic_cfg(n, real, u=(0.3, -0.2, 0.1), ho_factor=0.5)
+    B0 = random_modes_field(n, seed=4, amp=0.3)
+    cfg.dt = hall_dt(B0, (cfg.dx, cfg.dy, cfg.dz), 6, c_hall=0.8, cfl=0.3)
+    prog = Program(cfg)
+    nf = 2 if integrator == "CarpenterKennedy2N54" else 3
+    args = _device_args(prog, B0, nf, (0.3, -0.2, 0.1), 1.25)
+    with _Env(IEQ_FUSED_ZC=zc, IEQ_EMF_ZC=zc, IEQ_FUSED_SPLIT=split):
+        prog.step(integrator, 2, args)
+        torch.cuda.synchronize()
+    assert prog.last_step_fused
+    hb, _ = _oracle_steps(cfg, B0, integrator, 2, (0.3, -0.2, 0.1), 1.25)
+    e = _relerr(args[0][:, : cfg.num_nodes].double().cpu().numpy().T, hb[0][: cfg.num_nodes, :3].astype(np.float64))
+    assert e <= TOL[real], (zc, e)
+
+
+@pytest.mark.parametrize("ops,rate", [("4thOrder", 3.7), ("6thOrder", 5.6)])
+def test_fused_path_converges_to_exact_hall_solution(ops, rate):
+    """B0 + eps*b(x - v t), v = u + (m/rho) B0, b an ABC field (curl b = m b): fused ieq_step error vs the
+    exact solution falls at the operator's order (uniform u, rho = 0.8, Hall term on, no dissipation)."""
+    import torch
+
+    from inductioneq_amd.program import Program
+
+    u, rho, T = (0.2, -0.1, 0.15), 0.8, 0.4
+    errs = []
+    for n in (16, 32):
+        cfg = _periodic_cfg(n, ops=ops, diss=None, u=u)
+        B0 = beltrami_hall_field(n, 0.0, u=u, rho=rho)
+        dt = hall_dt(B0, (cfg.dx, cfg.dy, cfg.dz), int(ops[0]), c_hall=1.0 / rho, cfl=0.4)
+        steps = int(math.ceil(T / dt))
+        cfg.dt = T / steps
+        prog = Program(cfg)
+        args = _device_args(prog, B0, 2, u, rho)
+        prog.step("CarpenterKennedy2N54", steps, args)
+        torch.cuda.synchronize()
+        assert prog.last_step_fused
+        got = args[0][:, : cfg.num_nodes].cpu().numpy().T
+        exact = beltrami_hall_field(n, T, u=u, rho=rho)
+        errs.append(float(np.sqrt(((got - exact) ** 2).sum() / (exact ** 2).sum())))
+    got_rate = math.log(errs[0] / errs[1], 2)
+    print("exact Hall solution, %s: errors %s, rate %.2f" % (ops, errs, got_rate))
+    assert got_rate > rate, (errs, got_rate)
+
+
+@pytest.mark.parametrize("integrator", INTEGRATORS)
+def test_rk_step_list_takes_fused_path(integrator):
+    """ieq_run(repmat(RK_Step, 1, k)) on a qualifying program runs the fused kernels; b1, the other
+    registers, curlB_rho and time equal the reference kernel sequence (IEQ_NO_FUSED=1) to 1e-12."""
+    import torch
+
+    from inductioneq_amd.program import Program
+
+    n = (18, 20, 16)
+    u, rho = (0.3, -0.2, 0.1), 1.25
+    cfg = _periodic_cfg(n, u=u, ho_factor=0.5)
+    B0 = random_modes_field(n, seed=6, amp=0.3)
+    cfg.dt = hall_dt(B0, (cfg.dx, cfg.dy, cfg.dz), 6, c_hall=0.8, cfl=0.3)
+    prog = Program(cfg)
+    names, nf = rk_schedule(integrator, True, False)
+    k = 4
+    runs = []
+    for no_fused in ("0", "1"):
+        args = _device_args(prog, B0, nf, u, rho)
+        with _Env(IEQ_NO_FUSED=no_fused):
+            prog.run(names * k, args)
+            torch.cuda.synchronize()
+        assert prog.last_step_fused == (no_fused == "0")
+        runs.append(args)
+    fz, ref = runs
+    tol = 1e-12 * (20 if integrator == "SSPRK104" else 1)
+    for i in range(nf):
+        e = _relerr(fz[i][:, : cfg.num_nodes].cpu().numpy(), ref[i][:, : cfg.num_nodes].cpu().numpy())
+        # work registers of the low-storage schemes hold dt-scaled increments (cancellation)
+        assert e <= (tol if i == 0 else 1e-10), (i, e)
+    e = _relerr(fz[nf][:, : cfg.num_nodes].cpu().numpy(), ref[nf][:, : cfg.num_nodes].cpu().numpy())
+    assert e <= 1e-10, ("curlB_rho", e)
+    assert torch.equal(fz[-1], ref[-1])  # time[]
+
+
+def test_run_host_1000_step_list():
+    """MatCL-shaped call: ieq_run_host on host AoS arrays with a 1000-step CK2N54 list takes the fused
+    kernels and writes back the same b1 and time as the reference kernel sequence and the oracle."""
+    from inductioneq_amd.program import Program
+
+    n = 16
+    u, rho = (0.1, 0.2, -0.1), 1.0
+    cfg = _periodic_cfg(n, u=u)
+    B0 = random_modes_field(n, seed=2, amp=0.2)
+    cfg.dt = hall_dt(B0, (cfg.dx, cfg.dy, cfg.dz), 6, c_hall=1.0, cfl=0.3)
+    prog = Program(cfg)
+    names, nf = rk_schedule("CarpenterKennedy2N54", True, False)
+    k = 1000
+    outs = []
+    for no_fused in ("0", "1"):
+        hb = [to_aos4(B0, cfg.num_nodes_pad), np.zeros((cfg.num_nodes_pad, 4))]
+        hu = to_aos4(np.tile(np.asarray(u), (cfg.num_nodes, 1)), cfg.num_nodes_pad)
+        ht = np.zeros(2)
+        arrays = hb + [np.zeros((cfg.num_nodes_pad, 4)), hu, np.full(cfg.num_nodes_pad, rho), ht]
+        with _Env(IEQ_NO_FUSED=no_fused):
+            sec = prog.run_host(names * k, arrays)
+        assert prog.last_step_fused == (no_fused == "0")
+        assert sec > 0
+        outs.append(arrays)
+    hb_o, ht_o = _oracle_steps(cfg, B0, "CarpenterKennedy2N54", k, u, rho)
+    want = hb_o[0][: cfg.num_nodes, :3]
+    for arrays in outs:
+        e = _relerr(arrays[0][: cfg.num_nodes, :3], want)
+        print("1000-step list: rel err vs oracle %.3e" % e)
+        assert e <= 1e-11, e
+        assert abs(arrays[-1][0] - ht_o[0]) <= 1e-12 and arrays[-1][1] == arrays[-1][0]
+    assert _relerr(outs[0][0][:, :3], outs[1][0][:, :3]) <= 1e-11
+
+
+def test_step_uniform_equals_step():
+    import torch
+
+    from inductioneq_amd.program import Program
+
+    n = (20, 18, 16)
+    u, rho = (0.3, -0.2, 0.1), 1.25
+    for real in ("double", "float"):
+        cfg = _periodic_cfg(n, real, u=u)
+        B0 = random_modes_field(n, seed=8, amp=0.3)
+        cfg.dt = hall_dt(B0, (cfg.dx, cfg.dy, cfg.dz), 6, c_hall=0.8, cfl=0.3)
+        prog = Program(cfg)
+        a1, a2 = _device_args(prog, B0, 2, u, rho), _device_args(prog, B0, 2, u, rho)
+        prog.step("CarpenterKennedy2N54", 3, a1)
+        assert prog.last_step_fused
+        prog.step("CarpenterKennedy2N54", 3, a2, uniform=(u[0], u[1], u[2], rho))
+        assert prog.last_step_fused
+        torch.cuda.synchronize()
+        assert torch.equal(a1[0], a2[0]) and torch.equal(a1[-1], a2[-1])
+
+
+def test_non_qualifying_calls_run_reference_kernels():
+    import torch
+
+    from inductioneq_amd.program import Program
+
+    n = (16, 16, 16)
+    cfg = _periodic_cfg(n, u=(0.1, 0.0, 0.0))
+    B0 = random_modes_field(n, seed=3, amp=0.2)
+    cfg.dt = hall_dt(B0, (cfg.dx, cfg.dy, cfg.dz), 6, cfl=0.3)
+    prog = Program(cfg)
+    names, nf = rk_schedule("CarpenterKennedy2N54", True, False)
+    args = _device_args(prog, B0, nf, (0.1, 0.0, 0.0), 1.0)
+    prog.step("CarpenterKennedy2N54", 1, args)
+    assert prog.last_step_fused
+    # non-uniform velocity
+    args[3][0, 5] = 0.2
+    prog.step("CarpenterKennedy2N54", 1, args)
+    assert not prog.last_step_fused
+    # NaN in rho: never uniform
+    args = _device_args(prog, B0, nf, (0.1, 0.0, 0.0), 1.0)
+    args[4][7] = float("nan")
+    prog.step("CarpenterKennedy2N54", 1, args)
+    assert not prog.last_step_fused
+    # time-dependent velocity (calc_u in the list)
+    args = _device_args(prog, B0, nf, (0.1, 0.0, 0.0), 1.0)
+    names_u, _ = rk_schedule("CarpenterKennedy2N54", True, True)
+    prog.run(names_u, args)
+    assert not prog.last_step_fused
+    # a partial NDRange updates part of the grid only: reference kernels
+    prog.run(names, args, g=(8, 16, 16))
+    assert not prog.last_step_fused
+    # the list once, whole grid: fused
+    prog.run(names, args)
+    assert prog.last_step_fused
+    torch.cuda.synchronize()
