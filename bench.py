@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--integrator", default="CarpenterKennedy2N54")
     ap.add_argument("--cpu-steps", type=int, default=15, help="RK steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="hall128: skip the other measurements (256^3 FP64, 128^3 FP32, ensemble, adjoint, CG)")
     ap.add_argument("--workload", choices=["hall128", "ensemble64", "adjoint96", "rotation40", "cg192"],
                     default="hall128",
                     help="hall128: one periodic Hall solve (the headline metric); ensemble64: 64 batched FP32 solves "
@@ -124,68 +126,28 @@ def main():
     if a.workload == "cg192":
         bench_cg(a)
         return
-    import numpy as np
     import torch
 
-    from inductioneq_amd.problems import hall_dt, random_modes_field
-    from inductioneq_amd.program import Program
     from inductioneq_amd.schedules import STAGES_PER_STEP
-    from inductioneq_amd.settings import ProgramConfig
 
     n = a.n
-    L = 2 * math.pi
     real = "double" if a.precision == "fp64" else "float"
-    cfg = ProgramConfig(real=real, opset="6thOrder", nx=n, ny=n, nz=n, xmin=0, xmax=L, ymin=0, ymax=L, zmin=0, zmax=L,
-                        periodic=True, hall=True, dissipation="HIGH_ORDER", testcase="uniform_flow", w_size=256)
-    B0 = random_modes_field(n, seed=1)
-    cfg.dt = hall_dt(B0, (cfg.dx, cfg.dy, cfg.dz), 6, c_hall=1.0, cfl=0.5)
-    prog = Program(cfg)
-    prog.use_current_stream()
-    b1 = prog.vector()
-    b1[:, : cfg.num_nodes] = torch.as_tensor(B0.T, dtype=prog.torch_dtype, device="cuda")
-    b2 = prog.vector()
-    J, u, rho, tb = prog.vector(), prog.vector(), prog.scalar(1.0), prog.time_buffer()
-    args = [b1, b2, J, u, rho, tb]
     stages = STAGES_PER_STEP[a.integrator]
-
-    print("bench: warmup %d steps (n=%d, %s, %s)" % (a.warmup, n, a.precision, a.integrator), file=sys.stderr, flush=True)
-    prog.step(a.integrator, a.warmup, args)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record()
-    prog.step(a.integrator, a.steps, args)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    sec = ms * 1e-3
-    ncell = n ** 3
-    value = ncell * stages * a.steps / sec / 1e6
-    stage_s = sec / (stages * a.steps)
+    h = run_hall(n, real, a.integrator, a.steps, a.warmup)
+    cfg, B0 = h["cfg"], h["B0"]
     rb = 8 if real == "double" else 4
-    alg_bytes = 4 * 3 * rb * ncell  # read B, dB; write B', dB (3 components each)
-    achieved = alg_bytes / stage_s / 1e9
-    finite = bool(torch.isfinite(b1[:, : cfg.num_nodes]).all().item())
-    print("bench: %.3f ms for %d steps, %.1f us/stage, finite=%s" % (ms, a.steps, stage_s * 1e6, finite), file=sys.stderr,
-          flush=True)
     err64 = None
     if real == "float":
         # the FP32 state against the FP64 path (equal to the CPU oracle to 1e-12 per step) after the same steps
-        from dataclasses import replace
-
-        p64 = Program(replace(cfg, real="double"))
-        p64.use_current_stream()
-        c1 = p64.vector()
-        c1[:, : cfg.num_nodes] = torch.as_tensor(B0.T, dtype=torch.float64, device="cuda")
-        p64.step(a.integrator, a.warmup + a.steps, [c1, p64.vector(), p64.vector(), p64.vector(), p64.scalar(1.0),
-                                                    p64.time_buffer()])
-        torch.cuda.synchronize()
-        ref = c1[:, : cfg.num_nodes]
-        err64 = float(((b1[:, : cfg.num_nodes].double() - ref).abs().max() / ref.abs().max()).item())
+        err64 = fp32_vs_fp64(cfg, B0, a.integrator, a.warmup + a.steps, h["b1"])
         print("bench: fp32 vs fp64 after %d steps: max rel err %.3e" % (a.warmup + a.steps, err64), file=sys.stderr,
               flush=True)
-        del p64, c1, ref
+    del h["b1"]
+    torch.cuda.empty_cache()
 
+    secondary = {}
+    if not a.no_secondary:
+        secondary = run_secondary(a)
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(cfg, B0, a.integrator, a.cpu_steps, stages)
@@ -195,12 +157,12 @@ def main():
 
     out = {
         "metric": "Mcell-updates/s per RK stage",
-        "value": round(value, 3),
+        "value": h["value"],
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms / a.steps, 4),
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -209,20 +171,134 @@ def main():
         "config": {"workload": "hall_periodic_%d^3_6th_HOD_%s" % (n, a.integrator), "grid": [n, n, n],
                    "order": 6, "dissipation": "high_order", "u": [0, 0, 0], "hall_coefficient": 1.0, "dt": cfg.dt,
                    "stages_per_step": stages, "seed": 1},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "achieved": h["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": h["frac"],
                      "traffic": round(traffic["bytes_per_stage"]) if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
-                     "kernel": "one fused RK stage (k_emf + k_stage), %d B/cell algorithmic" % (4 * 3 * rb),
-                     "us_per_stage": round(stage_s * 1e6, 2)},
+                     "kernel": "one RK stage = one k_hall_stage launch, %d B/cell algorithmic (read B, dB; write B', dB)"
+                               % (4 * 3 * rb),
+                     "us_per_stage": h["us_per_stage"]},
         "cpu_baseline": cpu,
     }
     if err64 is not None:
         out["max_rel_err_vs_fp64"] = err64
+    out.update(secondary)
     print(json.dumps(out), flush=True)
 
 
+def run_hall(n, real, integrator, steps, warmup):
+    """The periodic Hall workload at n^3 (6th order + HO dissipation, u = 0, rho = 1, B0 = (0,0,1) + random modes,
+    seed 1): `steps` timed RK steps through ieq_step_uniform (inputs resident in HBM, no host synchronisation
+    inside the timed region)."""
+    import torch
+
+    from inductioneq_amd.problems import hall_dt, random_modes_field
+    from inductioneq_amd.program import Program
+    from inductioneq_amd.schedules import STAGES_PER_STEP
+    from inductioneq_amd.settings import ProgramConfig
+
+    L = 2 * math.pi
+    cfg = ProgramConfig(real=real, opset="6thOrder", nx=n, ny=n, nz=n, xmin=0, xmax=L, ymin=0, ymax=L, zmin=0, zmax=L,
+                        periodic=True, hall=True, dissipation="HIGH_ORDER", testcase="uniform_flow", w_size=256)
+    B0 = random_modes_field(n, seed=1)
+    cfg.dt = hall_dt(B0, (cfg.dx, cfg.dy, cfg.dz), 6, c_hall=1.0, cfl=0.5)
+    prog = Program(cfg)
+    prog.use_current_stream()
+    b1 = prog.vector()
+    b1[:, : cfg.num_nodes] = torch.as_tensor(B0.T, dtype=prog.torch_dtype, device="cuda")
+    args = [b1, prog.vector(), prog.vector(), prog.vector(), prog.scalar(1.0), prog.time_buffer()]
+    uni = (0.0, 0.0, 0.0, 1.0)
+    stages = STAGES_PER_STEP[integrator]
+    print("bench: warmup %d steps (n=%d, %s, %s)" % (warmup, n, real, integrator), file=sys.stderr, flush=True)
+    prog.step(integrator, warmup, args, uniform=uni)
+    torch.cuda.synchronize()
+    assert prog.last_step_fused, "the fused kernels did not run"
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    prog.step(integrator, steps, args, uniform=uni)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    sec = ms * 1e-3
+    ncell = n ** 3
+    stage_s = sec / (stages * steps)
+    rb = 8 if real == "double" else 4
+    alg_bytes = 4 * 3 * rb * ncell  # read B, dB; write B', dB (3 components each)
+    achieved = alg_bytes / stage_s / 1e9
+    finite = bool(torch.isfinite(b1[:, : cfg.num_nodes]).all().item())
+    print("bench: n=%d %s: %.3f ms for %d steps, %.2f us/stage, finite=%s" % (n, real, ms, steps, stage_s * 1e6, finite),
+          file=sys.stderr, flush=True)
+    if not finite:
+        raise RuntimeError("non-finite state after the timed steps")
+    return {"cfg": cfg, "B0": B0, "b1": b1, "value": round(ncell * stages * steps / sec / 1e6, 3),
+            "ms_per_step": round(ms / steps, 4), "us_per_stage": round(stage_s * 1e6, 2),
+            "achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4)}
+
+
+def fp32_vs_fp64(cfg, B0, integrator, nsteps, b1_32):
+    """Max relative difference of an FP32 state to the FP64 path after the same steps."""
+    from dataclasses import replace
+
+    import torch
+
+    from inductioneq_amd.program import Program
+
+    p64 = Program(replace(cfg, real="double"))
+    p64.use_current_stream()
+    c1 = p64.vector()
+    c1[:, : cfg.num_nodes] = torch.as_tensor(B0.T, dtype=torch.float64, device="cuda")
+    p64.step(integrator, nsteps, [c1, p64.vector(), p64.vector(), p64.vector(), p64.scalar(1.0), p64.time_buffer()],
+             uniform=(0.0, 0.0, 0.0, 1.0))
+    torch.cuda.synchronize()
+    ref = c1[:, : cfg.num_nodes]
+    return float(((b1_32[:, : cfg.num_nodes].double() - ref).abs().max() / ref.abs().max()).item())
+
+
+def run_secondary(a):
+    """The other BASELINE measurements, each on its own configuration: the Hall stage at 256^3 FP64 (beyond the
+    256 MB cache) and at 128^3 FP32, the batched ensemble against per-instance FP64 solves, the adjoint gradient
+    cost at 96^3 x 200 steps with 15 binomial snapshots, and CG divergence cleaning at 192^3."""
+    import torch
+
+    out = {}
+    h = run_hall(256, "double", "CarpenterKennedy2N54", 10, 2)
+    out["hall256_fp64"] = {k: h[k] for k in ("value", "us_per_stage", "achieved", "frac")}
+    out["hall256_fp64"]["unit"] = "Mcell-updates/s"
+    del h
+    torch.cuda.empty_cache()
+    h = run_hall(128, "float", "CarpenterKennedy2N54", 40, 5)
+    out["hall128_fp32"] = {k: h[k] for k in ("value", "us_per_stage", "achieved", "frac")}
+    out["hall128_fp32"]["max_rel_err_vs_fp64"] = fp32_vs_fp64(h["cfg"], h["B0"], "CarpenterKennedy2N54", 45, h["b1"])
+    del h
+    torch.cuda.empty_cache()
+    e = run_ensemble(a, 200)
+    out["ensemble64"] = {"value": e["value"], "unit": e["unit"], "speedup_vs_per_instance_fp64": e["speedup_vs_per_instance_fp64"],
+                         "max_rel_err_vs_fp64": e["max_rel_err_vs_fp64"], "roofline_frac": e["roofline"]["frac"],
+                         "per_instance_fp64": e["per_instance_fp64"], "launch": e["config"]["launch"]}
+    torch.cuda.empty_cache()
+    d = run_adjoint(a, 200, 96)
+    out["adjoint96"] = {"cost_x_forward_15_snapshots": d["binomial_checkpointing"]["cost_x_forward"],
+                        "forward_steps_15_snapshots": d["binomial_checkpointing"]["forward_steps"],
+                        "cost_x_forward_all_stages_stored": d["all_stages_stored"]["cost_x_forward"],
+                        "bitwise_same_gradient": d["all_stages_stored"]["bitwise_same_gradient_as_binomial"],
+                        "fd_rel_err_fp64": {k: v["rel_err"] for k, v in d["fd_check_fp64"].items()},
+                        "fp32_vs_fp64_grad_rel_err": d["fp32_vs_fp64_grad_rel_err"]}
+    torch.cuda.empty_cache()
+    c = run_cg(a, 192)
+    out["cg192"] = {"iterations_per_s": c["value"], "iterations": c["fp32"]["iterations"],
+                    "seconds_fp32_stencil_fp64_dots": c["fp32"]["seconds"], "seconds_fp64": c["fp64"]["seconds"],
+                    "divergence_norm_before": c["fp64"]["divergence_norm_initial"],
+                    "divergence_norm_after": c["fp64"]["divergence_norm_final"],
+                    "roofline_frac": c["roofline"]["frac"]}
+    return out
+
+
 def bench_ensemble(a):
+    print(json.dumps(run_ensemble(a, a.steps if a.steps != 40 else 200)), flush=True)
+
+
+def run_ensemble(a, steps):
     """64 periodic Hall solves (n in {32,48,64,96}, c_H in U(0.1, 2), B0 random modes with the member's seed,
     SSPRK33, dt from the Hall CFL 0.4), 6th order + HO dissipation: FP32 ensemble (one persistent launch per
     fused kernel per stage, hipGraph replay) vs the same solves one by one in FP64 (fused single-solve path)."""
@@ -239,7 +315,6 @@ def bench_ensemble(a):
     c_h = rng.uniform(0.1, 2.0, size=a.members)
     L = 2 * math.pi
     integ = "SSPRK33"
-    steps = a.steps if a.steps != 40 else 200
 
     def cfg_for(n, real, dt=1e-3):
         return ProgramConfig(real=real, opset="6thOrder", nx=n, ny=n, nz=n, xmin=0, xmax=L, ymin=0, ymax=L, zmin=0,
@@ -282,7 +357,7 @@ def bench_ensemble(a):
         args = [b1, prog.vector(), prog.vector(), prog.vector(), prog.vector(), prog.scalar(specs[i].rho),
                 prog.time_buffer()]
         torch.cuda.synchronize()
-        sec64 += prog.step(integ, steps, args, timed=True)
+        sec64 += prog.step(integ, steps, args, timed=True, uniform=(0.0, 0.0, 0.0, specs[i].rho))
         ref = b1[:, : n ** 3].double()
         got = ens.fields[i][0][:, : n ** 3].double()
         err = max(err, float(((got - ref).abs().max() / ref.abs().max()).item()))
@@ -314,10 +389,14 @@ def bench_ensemble(a):
         "max_rel_err_vs_fp64": err,
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_adjoint(a):
+    print(json.dumps(run_adjoint(a, a.steps if a.steps != 40 else 200, 96 if a.n == 128 else a.n)), flush=True)
+
+
+def run_adjoint(a, steps, n):
     """Gradient of 0.5 h^3 |B(T) - B_target|^2 after 200 SSPRK33 steps of the periodic Hall problem at 96^3
     (6th order + HO dissipation, u = (0.2, -0.1, 0.15), c_H = 1), w.r.t. B0 (FP32 state), u and c_H; B_target
     is the same solve from a different seed.  `value` is the gradient time (forward that keeps every stage input,
@@ -333,8 +412,6 @@ def bench_adjoint(a):
     from inductioneq_amd.program import Program
     from inductioneq_amd.settings import ProgramConfig
 
-    n = 96 if a.n == 128 else a.n
-    steps = a.steps if a.steps != 40 else 200
     snaps = 15
     L = 2 * math.pi
     u, ch = [0.2, -0.1, 0.15], 1.0
@@ -442,7 +519,7 @@ def bench_adjoint(a):
         "fd_check_fp64": r64["fd"],
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_rotation(a):
@@ -516,6 +593,10 @@ def bench_rotation(a):
 
 
 def bench_cg(a):
+    print(json.dumps(run_cg(a, 192 if a.n == 128 else a.n)), flush=True)
+
+
+def run_cg(a, n):
     """Divergence cleaning at 192^3 on [-1,1]^3 (4th-order SBP, WS_D0 Laplacian): B = the rotation40 field plus
     grad psi (Fourier |k| <= 6, seed 3), CG to relative residual 1e-10; FP32 stencils with FP64 dot products
     against all-FP64.  Reports iterations, time per iteration and the divergence norm before and after."""
@@ -526,7 +607,6 @@ def bench_cg(a):
     from inductioneq_amd.program import Program
     from inductioneq_amd.settings import ProgramConfig
 
-    n = 192 if a.n == 128 else a.n
     B = gaussian_rotation_field(n) + gradient_potential(n, seed=3)
     res = {}
     for real in ("double", "float"):
@@ -574,30 +654,49 @@ def bench_cg(a):
                      "kernel": "one CG iteration (k_cg_lap + k_cg_update, FP64 dot partials summed inside both), 40 B/cell algorithmic FP32"},
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(cfg, B0, integrator, nsteps, stages):
-    """Reference-schedule C oracle (AoS REAL4, per-node loops, curl + stage kernels) on host cores."""
+    """Reference-schedule C oracle (AoS REAL4, per-node loops, curl + stage kernels) on the host cores: `nsteps`
+    steps on every thread OpenMP is given, and one step on a single core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import Oracle
 
     from inductioneq_amd.problems import to_aos4
     from inductioneq_amd.schedules import rk_schedule
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    orc = Oracle(cfg, nthreads=threads)
     names, nf = rk_schedule(integrator, True, False)
-    b = [to_aos4(B0, cfg.num_nodes_pad, orc.dtype)] + [orc.vec() for _ in range(nf - 1)]
-    args = b + [orc.vec(), orc.vec(), orc.scal(1.0), orc.time()]
-    print("bench: cpu baseline %d step(s) on %d threads" % (nsteps, threads), file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    orc.run(names * nsteps, args)
-    dt = time.perf_counter() - t0
+
+    def timed(threads, k):
+        orc = Oracle(cfg, nthreads=threads)
+        b = [to_aos4(B0, cfg.num_nodes_pad, orc.dtype)] + [orc.vec() for _ in range(nf - 1)]
+        args = b + [orc.vec(), orc.vec(), orc.scal(1.0), orc.time()]
+        print("bench: cpu baseline %d step(s) on %d thread(s)" % (k, threads), file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        orc.run(names * k, args)
+        return time.perf_counter() - t0
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    dt = timed(threads, nsteps)
+    dt1 = timed(1, 1)
     v = cfg.num_nodes * stages * nsteps / dt / 1e6
     return {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
             "sample": "%d RK step(s) (%d stages) of the same %dx%dx%d workload, %.2f s" % (nsteps, stages * nsteps, cfg.nx,
-                                                                                        cfg.ny, cfg.nz, dt)}
+                                                                                        cfg.ny, cfg.nz, dt),
+            "cpu_model": cpu_model(), "logical_cpus_visible": os.cpu_count(),
+            "single_core": {"value": round(cfg.num_nodes * stages / dt1 / 1e6, 3), "cores": 1,
+                            "sample": "1 RK step (%d stages), %.2f s" % (stages, dt1)}}
 
 
 if __name__ == "__main__":

@@ -160,6 +160,17 @@ int ieq_step_uniform(ieq_program *prog, const char *integrator, long long nsteps
  * IEQ_NO_FUSED=1 in the environment keeps every call on the reference kernel sequence. */
 int ieq_last_step_path(const ieq_program *prog);
 
+/* FP64 diagnostics of the vector field b at time `time`, whatever REAL is, in one pass with every
+ * product and sum in double: out[0] = the SBP-norm energy sum fac |B|^2 (norm2,
+ * kernel_operator.cl:11-44), out[1] = sqrt(sum fac (div_D B)^2) (calc_div + norm2_S,
+ * kernel_operator.cl:98-155), out[2] = sqrt(sum fac |B - b_analytical(time)|^2) (analytical_b +
+ * norm2_diff, kernel_init.cl:36-48; NaN for a testcase without an analytic field), fac =
+ * prod_d dx_d / M_INV[row_d].  Replaces the host loop of reductions of
+ * compute_numerical_solution.m:69-89,218-250, which forms these sums in REAL.  The partial sums
+ * are added in a fixed order (deterministic).  Waits for the result. */
+int ieq_diagnostics(ieq_program *prog, const void *b, double time, double out[3]);
+
+
 /* Result of ieq_clean_div_cg. */
 typedef struct {
   double divergence_norm_initial; /* sqrt(r.r), r = div B on the BNODES interior (clean_div_CG.m:36-44) */
@@ -213,7 +224,8 @@ int ieq_ensemble_create(ieq_ensemble **ens, ieq_program *prog, const ieq_member 
  * 2N schemes, which must get an even nsteps).  b1 of every member holds the state on return. */
 int ieq_ensemble_step(ieq_ensemble *ens, long long nsteps, double *kernel_seconds);
 
-/* Chosen launch configuration: tile shape code (4: 64 x 4 cells, 8: 64 x 8, 32: 32 x 8),
+/* Chosen launch configuration: tile shape code (4: 64 x 4 cells, 8: 64 x 8, 32: 32 x 8 with the EMF
+ * in its own launch; 1: the one-kernel Hall stage, 32 x 8 in FP64 and 64 x 8 in FP32),
  * z-chunk, order (0 member order, 1 largest first; +2 when the step is replayed from a
  * hipGraph rather than launched directly, whichever timed faster). */
 int ieq_ensemble_config(const ieq_ensemble *ens, int *tile_shape, int *z_chunk, int *order);

@@ -22,7 +22,7 @@ FLAGS = ["--offload-arch=%s" % ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wn
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 SOURCES = ["ieq_fused.hip", "ieq_ensemble.hip", "ieq_hall_f64.hip", "ieq_hall_f32.hip", "ieq_generic.hip", "ieq_adjoint.hip",
-           "ieq_cg.hip", "ieq_capi.hip", "ieq_kernel_exports.hip"]
+           "ieq_cg.hip", "ieq_diag.hip", "ieq_capi.hip", "ieq_kernel_exports.hip"]
 
 
 def _compile(src: str) -> str:

@@ -15,6 +15,7 @@
 #include "ieq_adjoint.hpp"
 #include "ieq_cg.hpp"
 #include "ieq_device.hpp"
+#include "ieq_diag.hpp"
 #include "ieq_fused.hpp"
 #include "ieq_generic.hpp"
 
@@ -170,6 +171,7 @@ struct ieq_program {
   ieq::FusedWork* fused = nullptr;
   ieq::CgWork* cg = nullptr;
   int last_path = -1;  // path of the last ieq_run / ieq_step: 0 reference kernels, 1 fused kernels
+  double* diag = nullptr;  // ieq_diagnostics: 3 * kDiagBlocks partials + the 3 results
 
   bool has(const char* m) const { return modules.count(m) != 0; }
   int fail(int code, const std::string& msg) {
@@ -559,6 +561,7 @@ void ieq_program_release(ieq_program* prog) {
   if (!prog) return;
   ieq::fused_release(prog->fused);
   ieq::cg_release(prog->cg);
+  if (prog->diag) (void)hipFree(prog->diag);
   delete prog;
 }
 
@@ -738,6 +741,23 @@ int ieq_step_uniform(ieq_program* prog, const char* integrator, long long nsteps
 }
 
 int ieq_last_step_path(const ieq_program* prog) { return prog ? prog->last_path : -1; }
+
+int ieq_diagnostics(ieq_program* prog, const void* b, double time, double out[3]) {
+  if (!prog || !b || !out) return IEQ_ERR_ARGS;
+  if (!prog->diag) HIP_TRY(prog, hipMalloc(&prog->diag, (3 * ieq::kDiagBlocks + 3) * sizeof(double)));
+  double* res = prog->diag + 3 * ieq::kDiagBlocks;
+  const int analytic = prog->info.testcase != 7;  // uniform_flow has no analytic field
+  hipError_t e = prog->info.real_bytes == 8
+                     ? ieq::diag_launch<double>(prog->p, prog->info.opset, prog->stream, (const double*)b, time, analytic,
+                                                prog->diag, res)
+                     : ieq::diag_launch<float>(prog->p, prog->info.opset, prog->stream, (const float*)b, time, analytic,
+                                               prog->diag, res);
+  if (e != hipSuccess) return hip_fail(prog, e, "diagnostics launch");
+  HIP_TRY(prog, hipMemcpyAsync(out, res, 3 * sizeof(double), hipMemcpyDeviceToHost, prog->stream));
+  HIP_TRY(prog, hipStreamSynchronize(prog->stream));
+  return IEQ_OK;
+}
+
 
 }  // extern "C"
 

@@ -8,6 +8,7 @@
 #include "ieq_fused.hpp"
 #include "ieq_fused_tiles.hpp"
 #include "ieq_generic.hpp"
+#include "ieq_hall.hpp"
 #include "ieq_stencil.hpp"
 
 namespace ieq {
@@ -22,6 +23,7 @@ namespace ieq {
 struct Ensemble {
   int real_bytes = 8;
   int rr = 3, hall = 0, hod = 0, conv = 0, src_zero = 0;
+  bool hall1 = false;  // the one-kernel Hall stage applies (32-bit staged-halo offsets fit every member)
   StencilCoef<double> coef;  // central-row coefficients (converted to REAL per member)
   std::vector<ieq_member> m;
   std::vector<StageSpec> plan;
@@ -116,9 +118,27 @@ template <typename R> std::vector<FArgs<R>> ensemble_args(Ensemble& e) {
 }
 
 // Tile shapes by code, as for the single-grid launches: 4 -> 64 x 4, 8 -> 64 x 8, 32 -> 32 x 8
-// (32-wide tiles waste fewer lanes on members whose nx is not a multiple of 64).
-constexpr int ens_tx(int code) { return code == 32 ? 32 : 64; }
-constexpr int ens_ty(int code) { return code == 32 ? 8 : code; }
+// (32-wide tiles waste fewer lanes on members whose nx is not a multiple of 64); code 1 is the
+// one-kernel Hall stage (k_hall_stage_batched, E formed in the stage kernel) with its own tile.
+constexpr int kHallCode = 1;
+int ens_tx(int code, int real_bytes) {
+  if (code == kHallCode) {
+    int tx = 0, ty = 0;
+    if (real_bytes == 8) hall_tile_shape<double>(&tx, &ty);
+    else hall_tile_shape<float>(&tx, &ty);
+    return tx;
+  }
+  return code == 32 ? 32 : 64;
+}
+int ens_ty(int code, int real_bytes) {
+  if (code == kHallCode) {
+    int tx = 0, ty = 0;
+    if (real_bytes == 8) hall_tile_shape<double>(&tx, &ty);
+    else hall_tile_shape<float>(&tx, &ty);
+    return ty;
+  }
+  return code == 32 ? 8 : code;
+}
 
 std::vector<int4> ensemble_tiles(const Ensemble& e, int ty, int zc, int order) {
   std::vector<int> idx(e.m.size());
@@ -131,8 +151,8 @@ std::vector<int4> ensemble_tiles(const Ensemble& e, int ty, int zc, int order) {
   for (int i : idx) {
     const ieq_member& mb = e.m[i];
     for (int z = 0; z < mb.nz; z += zc)
-      for (int y = 0; y < mb.ny; y += ens_ty(ty))
-        for (int x = 0; x < mb.nx; x += ens_tx(ty)) t.push_back(make_int4(x, y, z, i));
+      for (int y = 0; y < mb.ny; y += ens_ty(ty, e.real_bytes))
+        for (int x = 0; x < mb.nx; x += ens_tx(ty, e.real_bytes)) t.push_back(make_int4(x, y, z, i));
   }
   return t;
 }
@@ -162,8 +182,13 @@ template <typename R, int RR, int TY, int TX> const void* ens_kernel_t(const Ens
   }
 }
 
-// Kernel of a plan entry (kind 3: EMF, otherwise the stage kernel) for the ensemble's settings.
+// Kernel of a plan entry (kind 3: EMF, otherwise the stage kernel) for the ensemble's settings;
+// null for the EMF entries under the one-kernel Hall stage.
 template <typename R> const void* ens_kernel(const Ensemble& e, int code, const StageSpec& st) {
+  if (code == kHallCode) {
+    if (st.kind == 3) return nullptr;
+    return hall_stage_kernel<R>(e.rr, stage_mode(st.kind == 4, st.P, st.Q, st.Y), e.hod, e.conv != 0, true);
+  }
   if (code == 32) {
     if (e.rr == 1) return ens_kernel_t<R, 1, 8, 32>(e, st);
     if (e.rr == 2) return ens_kernel_t<R, 2, 8, 32>(e, st);
@@ -187,9 +212,11 @@ template <typename R> hipError_t ens_launch_steps(Ensemble& e, hipStream_t s) {
     for (const StageSpec& st : e.plan) {
       const FArgs<R>* a = base + k * nm;
       ++k;
+      const void* kern = ens_kernel<R>(e, e.ty, st);
+      if (!kern) continue;  // EMF entry under the one-kernel Hall stage
       void* kargs[3] = {(void*)&a, (void*)&e.dtiles, (void*)&e.ntiles};
       const int grid = st.kind == 3 ? e.grid_emf : e.grid_stage;
-      hipError_t r = hipLaunchKernel(ens_kernel<R>(e, e.ty, st), dim3(grid), dim3(ens_tx(e.ty), ens_ty(e.ty)), kargs, 0, s);
+      hipError_t r = hipLaunchKernel(kern, dim3(grid), dim3(ens_tx(e.ty, e.real_bytes), ens_ty(e.ty, e.real_bytes)), kargs, 0, s);
       if (r != hipSuccess) return r;
     }
   return hipSuccess;
@@ -219,8 +246,9 @@ template <typename R> int ens_configure(Ensemble& e, int ty, int zc, int order, 
   // (the smallest occupancy over the plan's stage kernels)
   int occ_emf = 1, occ_stage = 1 << 30;
   const StageSpec emf_spec{3, -1, -1, -1, 0, 0, 0, 0};
-  const int nthreads = ens_tx(ty) * ens_ty(ty);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_emf, ens_kernel<R>(e, ty, emf_spec), nthreads, 0) != hipSuccess)
+  const int nthreads = ens_tx(ty, e.real_bytes) * ens_ty(ty, e.real_bytes);
+  const void* kemf = ens_kernel<R>(e, ty, emf_spec);
+  if (kemf && hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_emf, kemf, nthreads, 0) != hipSuccess)
     return err = "occupancy query failed", -4;
   for (const StageSpec& st : e.plan) {
     if (st.kind == 3) continue;
@@ -297,10 +325,14 @@ template <typename R> int ens_tune(Ensemble& e, std::string& err) {
   hipEvent_t t0 = sc.ev[0], t1 = sc.ev[1], t2 = sc.ev[2];
   float best = 1e30f;
   int bty = 8, bzc = 8, bord = 1;
-  const int tys[3] = {4, 8, 32}, zcs[3] = {4, 8, 16};
+  const int tys[4] = {kHallCode, 4, 8, 32}, zcs[3] = {4, 8, 16};
+  // IEQ_ENS_SHAPE=<code> restricts the search to one tile shape (1, 4, 8 or 32)
+  const char* force = std::getenv("IEQ_ENS_SHAPE");
+  const int forced = force && *force ? std::atoi(force) : 0;
   for (int ty : tys)
     for (int zc : zcs)
       for (int ord = 0; ord < 2; ++ord) {
+        if ((ty == kHallCode && !e.hall1) || (forced && ty != forced)) continue;
         int rc = ens_configure<R>(e, ty, zc, ord, err);
         if (rc) return rc;
         hipError_t r = ens_launch_steps<R>(e, e.stream);  // warm-up
@@ -382,6 +414,9 @@ int ensemble_create(const Params& p, const ieq_program_info& info, const ieq_mem
     if (mb.ux != 0.0 || mb.uy != 0.0 || mb.uz != 0.0) e->conv = 1;
     e->m.push_back(mb);
   }
+  e->hall1 = e->hall != 0;
+  for (const ieq_member& mb : e->m)
+    if ((long long)3 * ieq_member_stride(mb.nx, mb.ny, mb.nz) * info.real_bytes >= (1LL << 32)) e->hall1 = false;
   e->steps_per_graph = (e->plan.size() > 0 && e->plan.back().kind == 4 && (e->plan.size() / (p.hall ? 2 : 1)) % 2) ? 2 : 1;
   int dev = 0;
   hipDeviceProp_t prop;

@@ -370,7 +370,8 @@ template <typename R> __global__ __launch_bounds__(BX* BY) void k_axpy(Params p,
 
 // Group reductions: output[g] = sum over the W_SIZE nodes of group g of
 //   which 0: fac*|B|^2 (norm2), 1: fac*|B1-B2|^2 (norm2_diff), 2: fac*s^2 (norm2_S), 3: a*b (dot_product)
-// with fac the SBP norm weight prod_d dx_d / M_INV[row_d] (kernel_operator.cl:11-137).
+// with fac the SBP norm weight prod_d dx_d / M_INV[row_d] (kernel_operator.cl:11-137), accumulated in
+// double.
 template <typename R, int OPS>
 __global__ __launch_bounds__(256) void k_group_reduce(Params p, const R* a, const R* b, R* out, int which, long long wsize,
                                                       long long nnodes) {
@@ -383,19 +384,29 @@ __global__ __launch_bounds__(256) void k_group_reduce(Params p, const R* a, cons
   for (long long l = threadIdx.x; l < wsize; l += blockDim.x) {
     long long gid = g0 + l;
     if (gid >= nnodes) continue;  // padding nodes hold zeros
+    // every product formed in double (FP32 programs widen the nodal values first); only the group
+    // partial is rounded to REAL, as the reference's output array is REAL
     double v;
     if (which == 3) {
-      v = (double)(a[gid] * b[gid]);
+      v = (double)a[gid] * (double)b[gid];
     } else {
       uint32_t iz = (uint32_t)(gid / nxy), iy = (uint32_t)((gid - iz * nxy) / p.nx), ix = (uint32_t)(gid - iy * p.nx - iz * nxy);
-      R fac = ((R)p.dx / (R)O::minv(bound_row(ix, p.nx, O::NB, p.periodic))) *
-              ((R)p.dy / (R)O::minv(bound_row(iy, p.ny, O::NB, p.periodic))) *
-              ((R)p.dz / (R)O::minv(bound_row(iz, p.nz, O::NB, p.periodic)));
-      R q;
-      if (which == 0) { V3<R> f = ld3(a, S, gid); q = f.x * f.x + f.y * f.y + f.z * f.z; }
-      else if (which == 1) { V3<R> f = ld3(a, S, gid) - ld3(b, S, gid); q = f.x * f.x + f.y * f.y + f.z * f.z; }
-      else { R f = a[gid]; q = f * f; }
-      v = (double)(fac * q);
+      const double fac = (p.dx / O::minv(bound_row(ix, p.nx, O::NB, p.periodic))) *
+                         (p.dy / O::minv(bound_row(iy, p.ny, O::NB, p.periodic))) *
+                         (p.dz / O::minv(bound_row(iz, p.nz, O::NB, p.periodic)));
+      double q;
+      if (which == 0) {
+        const double fx = a[gid], fy = a[S + gid], fz = a[2 * S + gid];
+        q = fx * fx + fy * fy + fz * fz;
+      } else if (which == 1) {
+        const double fx = (double)a[gid] - (double)b[gid], fy = (double)a[S + gid] - (double)b[S + gid],
+                     fz = (double)a[2 * S + gid] - (double)b[2 * S + gid];
+        q = fx * fx + fy * fy + fz * fz;
+      } else {
+        const double f = a[gid];
+        q = f * f;
+      }
+      v = fac * q;
     }
     acc += v;
   }

@@ -7,9 +7,9 @@
 
 namespace ieq {
 
-// Stage kernel for the arguments (update form from a.mode / P / Q / Y, convection when a.conv),
-// single-grid or batched over a tile list (int4 {x0, y0, z0, member}, FArgs per member).
-template <typename R> const void* hall_stage_kernel(int rr, const FArgs<R>& a, bool hod, bool batched);
+// Stage kernel for an update form (stage_mode), dissipation and convection, single-grid or
+// batched over a tile list (int4 {x0, y0, z0, member}, FArgs per member).
+template <typename R> const void* hall_stage_kernel(int rr, int mode, bool hod, bool conv, bool batched);
 // Tile shape (threads per workgroup = tx * ty) of the kernels above.
 template <typename R> void hall_tile_shape(int* tx, int* ty);
 template <typename R> void launch_hall_stage(int rr, const FArgs<R>& a, bool hod, hipStream_t s);

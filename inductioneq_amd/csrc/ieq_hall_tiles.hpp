@@ -24,10 +24,17 @@
 namespace ieq {
 namespace {
 
+// Extra LDS row elements (bank spreading of the scattered halo accesses); 0 by default.
+#ifndef IEQ_HALL_PADB
+#define IEQ_HALL_PADB 0
+#endif
+#ifndef IEQ_HALL_PADE
+#define IEQ_HALL_PADE 0
+#endif
 template <int RR, int TX, int TY> struct HallGeom {
   static constexpr int NT = TX * TY;
-  static constexpr int BX = TX + 4 * RR, BY = TY + 4 * RR;  // B plane with a 2*RR halo
-  static constexpr int EX = TX + 2 * RR, EY = TY + 2 * RR;  // E plane with an RR halo
+  static constexpr int BX = TX + 4 * RR + IEQ_HALL_PADB, BY = TY + 4 * RR;  // B plane with a 2*RR halo
+  static constexpr int EX = TX + 2 * RR + IEQ_HALL_PADE, EY = TY + 2 * RR;  // E plane with an RR halo
   static constexpr int NXH = RR * TY, NYH = RR * TX;        // E-halo columns per x side / per y side
   static constexpr int NH = 2 * (NXH + NYH);                // E-halo columns
   // B nodes beyond the E region that the x/y derivatives of J at E-halo nodes reach: the four
@@ -39,10 +46,11 @@ template <int RR, int TX, int TY> struct HallGeom {
   static_assert(NH <= NT, "one E-halo column per thread");
 };
 
-template <typename R, int RR, int TX, int TY> struct HallLds {
+// NBUF = 1: single planes, two barriers per z-step; NBUF = 2: double-buffered planes, one barrier.
+template <typename R, int RR, int TX, int TY, int NBUF> struct HallLds {
   using G = HallGeom<RR, TX, TY>;
-  R sB[3][G::BY][G::BX];
-  R sE[3][G::EY][G::EX];
+  R sB[NBUF][3][G::BY][G::BX];
+  R sE[NBUF][3][G::EY][G::EX];
 };
 
 // E-halo column h of a tile: offset (lx, ly) from the tile origin.
@@ -117,13 +125,17 @@ __device__ __forceinline__ void emf_at(const FArgs<R>& a, R (*sB)[BY][BX], int b
 // Stage kernel body.  MODE as in stage_tile: 0 out = cP*P + cQ*Q + cD*RHS;  1 (2N) b2 = cP*b2 +
 // cD*RHS, out = Y + cQ*b2;  2 as 0 with Q either Y or absent (only P loaded);  3 as 0 with P
 // either Y or absent and no Q.
-template <typename R, int RR, int TX, int TY, bool HOD, bool CONV, int MODE, int PD>
-__device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y0, int z0, HallLds<R, RR, TX, TY>& L) {
+// SK = 0: iteration o stages B plane pe = o + RR, forms E(pe), then (second barrier) the x/y
+// derivatives of E on pe.  SK = 1: the LDS planes are double-buffered and iteration o stages
+// pe = o + RR + 1 and forms E(pe) while it differentiates E(pe - 1) from the other buffer, so a
+// z-step needs one barrier; the windows are one plane longer.
+template <typename R, int RR, int TX, int TY, bool HOD, bool CONV, int MODE, int PD, int SK>
+__device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y0, int z0, HallLds<R, RR, TX, TY, 1 + SK>& L) {
   using G = HallGeom<RR, TX, TY>;
   constexpr int Q = 2 * RR + 1;
+  constexpr int QB = Q + SK;              // B window: planes o .. q = o + 2RR + SK
+  constexpr int NRX = RR + 1 + SK;        // pending RHS: planes o .. pe
   constexpr int NPW = MODE == 0 ? 6 : MODE == 3 ? 0 : 3;  // pointwise loads: P (b2), [Q]
-  auto& sB = L.sB;
-  auto& sE = L.sE;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
   const bool active = x0 + tx < a.nx && y0 + ty < a.ny;
   const long long S = a.S;
@@ -141,7 +153,7 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
   int hlx, hly;
   halo_col<RR, TX, TY>(has_h ? tid : G::NH - 1, hlx, hly);
   const unsigned hcolb = (unsigned)(wrap(x0 + hlx, a.nx) + a.nx * wrap(y0 + hly, a.ny)) * RB;
-  // staging items of this thread: LDS index into sB (flattened) and byte offset (component included)
+  // staging items of this thread: LDS index into a B plane (flattened) and byte offset (component included)
   int sidx[G::K];
   unsigned soff[G::K];
   bool sown[G::K];
@@ -154,17 +166,16 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
     sidx[k] = (c * G::BY + ly + 2 * RR) * G::BX + lx + 2 * RR;
     soff[k] = (unsigned)(wrap(x0 + lx, a.nx) + a.nx * wrap(y0 + ly, a.ny)) * RB + (unsigned)(c * S) * RB;
   }
-  R* sBf = &sB[0][0][0];
 
-  R qb[3][Q];          // B of the own column: planes pe-RR .. pe+RR
+  R qb[3][QB];         // B of the own column: planes o .. q
   R qh[2][Q];          // B_x, B_y of the E-halo column: planes pe-RR .. pe+RR
-  R qe[2][Q];          // E_x, E_y of the own column: planes o-RR .. o+RR = pe-2RR .. pe
-  R rx[3][RR + 1];     // RHS without the z-derivatives of E, planes o .. pe
+  R qe[2][QB];         // E_x, E_y of the own column: planes o-RR .. pe
+  R rx[3][NRX];        // RHS without the z-derivatives of E, planes o .. pe
   R cs[PD][5];         // column loads in flight (B of the own column, B_x/B_y of the halo column)
   R pw[PD][NPW > 0 ? NPW : 1];  // pointwise loads in flight (output plane)
   R st[G::K], shz;     // staging loads in flight (B halo of the next plane, B_z of the halo column)
 
-  const int o0 = z0 - 2 * RR;
+  const int o0 = z0 - 2 * RR - SK;
   const int zlast = zend - 1 + 2 * RR;  // last B plane the chunk needs
   auto colld = [&](R* d, int pl) {
     const unsigned pz = (unsigned)wrap(min(pl, zlast), a.nz) * plb;
@@ -189,57 +200,115 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
     for (int k = 0; k < G::K; ++k) st[k] = ldo(B.c[0], soff[k] + pz);
     shz = ldo(B.c[2], hcolb + pz);
   };
-  // prologue: planes o0 .. o0 + 2RR - 1 of the windows (the first iteration brings plane o0 + 2RR)
+  // prologue: the windows' planes before the first iteration's heads (plane o0 + 2RR + SK)
 #pragma unroll
-  for (int j = 0; j < 2 * RR; ++j) {
+  for (int j = 0; j < QB - 1; ++j) {
     const unsigned pz = (unsigned)wrap(o0 + j, a.nz) * plb;
 #pragma unroll
     for (int c = 0; c < 3; ++c) qb[c][j] = ldo(B.c[c], colb + pz);
+  }
+#pragma unroll
+  for (int j = 0; j < Q - 1; ++j) {
+    const unsigned pz = (unsigned)wrap(o0 + SK + j, a.nz) * plb;
     qh[0][j] = ldo(B.c[0], hcolb + pz);
     qh[1][j] = ldo(B.c[1], hcolb + pz);
   }
   __builtin_amdgcn_sched_barrier(0);
-  colld(cs[0], o0 + 2 * RR);  // as issued by iteration o0 - PD
+  colld(cs[0], o0 + 2 * RR + SK);  // as issued by iteration o0 - PD
   ptld(pw[0], o0);
   __builtin_amdgcn_sched_barrier(0);
-  stld(o0 + RR);  // as issued by iteration o0 - 1
+  stld(o0 + RR + SK);  // as issued by iteration o0 - 1
   if constexpr (PD == 2) {
-    colld(cs[1], o0 + 2 * RR + 1);
+    colld(cs[1], o0 + 2 * RR + SK + 1);
     ptld(pw[1], o0 + 1);
   }
   __builtin_amdgcn_sched_barrier(0);
 
   auto body = [&](auto slot, int o) {
     constexpr int SL = decltype(slot)::value;
-    const int pe = o + RR;
-    // window heads: plane q = o + 2RR
+    const int pe = o + RR + SK;
+    const int buf = SK ? (PD == 2 ? SL : ((o - o0) & 1)) : 0;
+    auto& sB = L.sB[buf];
+    auto& sE = L.sE[buf];
+    // window heads: plane q = pe + RR
 #pragma unroll
-    for (int c = 0; c < 3; ++c) qb[c][Q - 1] = cs[SL][c];
+    for (int c = 0; c < 3; ++c) qb[c][QB - 1] = cs[SL][c];
     qh[0][Q - 1] = cs[SL][3];
     qh[1][Q - 1] = cs[SL][4];
     // B plane pe into LDS: own column, E-halo column, staged halo items
     const int cx = tx + 2 * RR, cy = ty + 2 * RR;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) sB[c][cy][cx] = qb[c][RR];
+    for (int c = 0; c < 3; ++c) sB[c][cy][cx] = qb[c][RR + SK];
     if (has_h) {
       sB[0][hly + 2 * RR][hlx + 2 * RR] = qh[0][RR];
       sB[1][hly + 2 * RR][hlx + 2 * RR] = qh[1][RR];
       sB[2][hly + 2 * RR][hlx + 2 * RR] = shz;
     }
+    R* sBf = &sB[0][0][0];
 #pragma unroll
     for (int k = 0; k < G::K; ++k)
       if (sown[k]) sBf[sidx[k]] = st[k];
     lds_barrier();
-    // E on plane pe: own column and E-halo column
+    // E on plane pe at the own column, and the dissipation and convection terms of plane pe.  Each
+    // x/y row of B is read from LDS once and feeds J (the derivatives curl B needs), the dissipation
+    // sum and the convection derivative; the z rows come from the register window.
+    const bool rhs_pe = pe >= z0 && pe < zend;
+    R r[3] = {R(0), R(0), R(0)};
     {
-      R Ex, Ey, Ez;
-      emf_at<R, RR, G::BY, G::BX>(a, sB, cx, cy, deriv<R, RR>(a.c, qb[0]), deriv<R, RR>(a.c, qb[1]), qb[0][RR], qb[1][RR],
-                                  qb[2][RR], irho, Ex, Ey, Ez);
+      R dB[3][3];   // undivided derivative sums: component c along direction d
+      R hsum[3];    // dissipation sums per component
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const bool for_j = c != d;  // curl B uses every derivative across components
+          if (!(for_j || CONV || HOD)) continue;
+          R v[Q];
+#pragma unroll
+          for (int k = 0; k < Q; ++k)
+            v[k] = k == RR ? qb[c][RR + SK]
+                           : d == 0 ? sB[c][cy][cx - RR + k] : d == 1 ? sB[c][cy - RR + k][cx] : qb[c][SK + k];
+          if (for_j || CONV) dB[c][d] = deriv<R, RR>(a.c, v);
+          if constexpr (HOD) {
+            if (rhs_pe) hsum[c] = d == 0 ? hodsum<R, RR>(a.c, v) : hsum[c] + hodsum<R, RR>(a.c, v);
+          }
+        }
+      }
+      // J = curl_D B / rho, E = J x B  (SBP_operator.cl:126-135, kernel_time_integrator.cl:147)
+      const R Jx = (dB[2][1] * a.inv_h[1] - dB[1][2] * a.inv_h[2]) * irho;
+      const R Jy = (dB[0][2] * a.inv_h[2] - dB[2][0] * a.inv_h[0]) * irho;
+      const R Jz = (dB[1][0] * a.inv_h[0] - dB[0][1] * a.inv_h[1]) * irho;
+      const R Bx = qb[0][RR + SK], By = qb[1][RR + SK], Bz = qb[2][RR + SK];
+      const R Ex = Jy * Bz - Jz * By, Ey = Jz * Bx - Jx * Bz, Ez = Jx * By - Jy * Bx;
       sE[0][ty + RR][tx + RR] = Ex;
       sE[1][ty + RR][tx + RR] = Ey;
       sE[2][ty + RR][tx + RR] = Ez;
-      qe[0][Q - 1] = Ex;
-      qe[1][Q - 1] = Ey;
+      qe[0][QB - 1] = Ex;
+      qe[1][QB - 1] = Ey;
+      if (rhs_pe) {
+        if constexpr (CONV) {
+          //   sum_d (2/dx_d) 0.5 ([ZERO] u (D_d B)_d - u_d (D_d B))
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              const R f = a.two_h[d] * R(0.5);
+              r[c] = r[c] - f * (a.u[d] * dB[c][d]);
+              if (c == d && a.src_zero) {
+#pragma unroll
+                for (int e = 0; e < 3; ++e) r[e] = r[e] + f * (a.u[e] * dB[c][d]);
+              }
+            }
+          }
+        }
+        if constexpr (HOD) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) r[c] = r[c] + hsum[c];
+        }
+      }
+    }
+    // E at the E-halo column
+    {
       R hx, hy, hz;
       emf_at<R, RR, G::BY, G::BX>(a, sB, hlx + 2 * RR, hly + 2 * RR, deriv<R, RR>(a.c, qh[0]), deriv<R, RR>(a.c, qh[1]),
                                   qh[0][RR], qh[1][RR], shz, irho, hx, hy, hz);
@@ -249,57 +318,31 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
         sE[2][hly + RR][hlx + RR] = hz;
       }
     }
-    // dissipation and convection on plane pe (x/y stencils from LDS, z stencil from the window)
-    R r[3] = {R(0), R(0), R(0)};
-    const bool rhs_pe = pe >= z0 && pe < zend;
-    if (rhs_pe) {
-      if constexpr (CONV || HOD) {
-        R v[Q];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          R h = R(0);
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-#pragma unroll
-            for (int k = 0; k < Q; ++k)
-              v[k] = k == RR ? qb[c][RR] : d == 0 ? sB[c][cy][cx - RR + k] : d == 1 ? sB[c][cy - RR + k][cx] : qb[c][k];
-            if constexpr (CONV) {
-              //   sum_d (2/dx_d) 0.5 ([ZERO] u (D_d B)_d - u_d (D_d B))
-              const R der = deriv<R, RR>(a.c, v);
-              const R f = a.two_h[d] * R(0.5);
-              r[c] = r[c] - f * (a.u[d] * der);
-              if (c == d && a.src_zero) {
-#pragma unroll
-                for (int e = 0; e < 3; ++e) r[e] = r[e] + f * (a.u[e] * der);
-              }
-            }
-            if constexpr (HOD) h = d == 0 ? hodsum<R, RR>(a.c, v) : h + hodsum<R, RR>(a.c, v);
-          }
-          if constexpr (HOD) r[c] = r[c] + h;
-        }
-      }
-    }
-    lds_barrier();
-    if (rhs_pe) {  // -curl_D E, x/y terms
+    for (int c = 0; c < 3; ++c) rx[c][NRX - 1] = r[c];
+    // -curl_D E, x/y terms of plane pe - SK (this plane after a second barrier, the previous plane
+    // from the other buffer)
+    if constexpr (SK == 0) lds_barrier();
+    const int pd = pe - SK;
+    if (pd >= z0 && pd < zend) {
+      auto& sEd = L.sE[SK ? buf ^ 1 : 0];
       R v[Q];
 #pragma unroll
-      for (int k = 0; k < Q; ++k) v[k] = sE[1][ty + RR][tx + k];
+      for (int k = 0; k < Q; ++k) v[k] = sEd[1][ty + RR][tx + k];
       const R dxEy = deriv<R, RR>(a.c, v);
 #pragma unroll
-      for (int k = 0; k < Q; ++k) v[k] = sE[2][ty + RR][tx + k];
+      for (int k = 0; k < Q; ++k) v[k] = sEd[2][ty + RR][tx + k];
       const R dxEz = deriv<R, RR>(a.c, v);
 #pragma unroll
-      for (int k = 0; k < Q; ++k) v[k] = sE[0][ty + k][tx + RR];
+      for (int k = 0; k < Q; ++k) v[k] = sEd[0][ty + k][tx + RR];
       const R dyEx = deriv<R, RR>(a.c, v);
 #pragma unroll
-      for (int k = 0; k < Q; ++k) v[k] = sE[2][ty + k][tx + RR];
+      for (int k = 0; k < Q; ++k) v[k] = sEd[2][ty + k][tx + RR];
       const R dyEz = deriv<R, RR>(a.c, v);
-      r[0] = r[0] + a.inv_h[1] * (-dyEz);
-      r[1] = r[1] + a.inv_h[0] * dxEz;
-      r[2] = r[2] + (a.inv_h[0] * (-dxEy) + a.inv_h[1] * dyEx);
+      rx[0][RR] = rx[0][RR] + a.inv_h[1] * (-dyEz);
+      rx[1][RR] = rx[1][RR] + a.inv_h[0] * dxEz;
+      rx[2][RR] = rx[2][RR] + (a.inv_h[0] * (-dxEy) + a.inv_h[1] * dyEx);
     }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) rx[c][RR] = r[c];
     // output plane o: z-derivatives of E, then the RK update
     if (active && o >= z0 && o < zend) {
       R rhs[3] = {rx[0][0], rx[1][0], rx[2][0]};
@@ -343,20 +386,22 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
     // next loads: staging of plane pe + 1, then the columns of plane q + PD and the pointwise
     // operands of output plane o + PD
     stld(pe + 1);
-    colld(cs[SL], o + 2 * RR + PD);
+    colld(cs[SL], pe + RR + PD);
     ptld(pw[SL], o + PD);
 #pragma unroll
-    for (int j = 0; j < Q - 1; ++j) {
+    for (int j = 0; j < QB - 1; ++j) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) qb[c][j] = qb[c][j + 1];
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        qh[c][j] = qh[c][j + 1];
-        qe[c][j] = qe[c][j + 1];
-      }
+      for (int c = 0; c < 2; ++c) qe[c][j] = qe[c][j + 1];
     }
 #pragma unroll
-    for (int k = 0; k < RR; ++k) {
+    for (int j = 0; j < Q - 1; ++j) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) qh[c][j] = qh[c][j + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < NRX - 1; ++k) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) rx[c][k] = rx[c][k + 1];
     }
@@ -374,27 +419,34 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
 #define IEQ_HALL_WAVES32 4
 #endif
 // Planes of column loads in flight: two in FP64 (2 waves per SIMD), one in FP32.
-template <typename R> constexpr int hall_pd() { return sizeof(R) == 8 ? 2 : 1; }
+#ifndef IEQ_HALL_PD64
+#define IEQ_HALL_PD64 2
+#endif
+template <typename R> constexpr int hall_pd() { return sizeof(R) == 8 ? IEQ_HALL_PD64 : 1; }
+// One barrier per z-step with double-buffered LDS planes (1) or two barriers and single planes (0).
+#ifndef IEQ_HALL_SKEW
+#define IEQ_HALL_SKEW 0
+#endif
 
 template <typename R, int RR, int TX, int TY, bool HOD, bool CONV, int MODE>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? IEQ_HALL_WAVES32 : IEQ_HALL_WAVES, 8))) void k_hall_stage(
     FArgs<R> a) {
-  __shared__ HallLds<R, RR, TX, TY> L;
+  __shared__ HallLds<R, RR, TX, TY, 1 + IEQ_HALL_SKEW> L;
   int x0, y0, z0;
   tile_origin<TY, TX>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
-  hall_stage_tile<R, RR, TX, TY, HOD, CONV, MODE, hall_pd<R>()>(a, x0, y0, z0, L);
+  hall_stage_tile<R, RR, TX, TY, HOD, CONV, MODE, hall_pd<R>(), IEQ_HALL_SKEW>(a, x0, y0, z0, L);
 }
 
 // Persistent batched form over a tile list covering every ensemble member.
 template <typename R, int RR, int TX, int TY, bool HOD, bool CONV, int MODE>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? IEQ_HALL_WAVES32 : IEQ_HALL_WAVES, 8))) void k_hall_stage_batched(
     const FArgs<R>* __restrict__ args, const int4* __restrict__ tiles, int ntiles) {
-  __shared__ HallLds<R, RR, TX, TY> L;
+  __shared__ HallLds<R, RR, TX, TY, 1 + IEQ_HALL_SKEW> L;
   const int chunk = (ntiles + kXcds - 1) / kXcds, slots = gridDim.x / kXcds;  // gridDim.x is a multiple of 8
   const int first = (blockIdx.x % kXcds) * chunk, last = min(first + chunk, ntiles);
   for (int t = first + (int)blockIdx.x / kXcds; t < last; t += slots) {
     const int4 d = tiles[t];
-    hall_stage_tile<R, RR, TX, TY, HOD, CONV, MODE, hall_pd<R>()>(args[d.w], d.x, d.y, d.z, L);
+    hall_stage_tile<R, RR, TX, TY, HOD, CONV, MODE, hall_pd<R>(), IEQ_HALL_SKEW>(args[d.w], d.x, d.y, d.z, L);
     lds_barrier();
   }
 }

@@ -94,6 +94,10 @@ class Ensemble:
         ty, zc, order = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _capi.check(self.lib.ieq_ensemble_config(self.handle, ctypes.byref(ty), ctypes.byref(zc), ctypes.byref(order)),
                     self.program.handle)
-        tile = {4: (64, 4), 8: (64, 8), 32: (32, 8)}.get(ty.value, (64, ty.value))  # tile shape code -> (x, y)
-        return {"tile": "%dx%d" % tile, "tile_rows": tile[1], "z_chunk": zc.value,
+        if ty.value == 1:  # the one-kernel Hall stage (E formed in the stage kernel)
+            tile, name = ((32, 8) if self.program.info.real_bytes == 8 else (64, 8)), "hall"
+        else:
+            tile = {4: (64, 4), 8: (64, 8), 32: (32, 8)}.get(ty.value, (64, ty.value))  # tile shape code -> (x, y)
+            name = "%dx%d" % tile
+        return {"tile": name, "tile_rows": tile[1], "z_chunk": zc.value,
                 "order": ("member", "largest_first")[order.value & 1], "graph": bool(order.value & 2)}

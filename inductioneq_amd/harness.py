@@ -51,6 +51,10 @@ class HipExecutor:
     def step(self, integrator, nsteps, args, variable_u=False):
         return self.prog.step(integrator, nsteps, args, variable_u=variable_u)
 
+    def clean_div(self, b, threshold, max_iterations):
+        """clean_div.m through the library's CG (ieq_clean_div_cg, absolute threshold like the reference)."""
+        return self.prog.clean_div_cg(b, abs_threshold=threshold, max_iterations=max_iterations)
+
     def reduce(self, name, *fields):
         return self.prog.reduce(name, *fields)
 
@@ -106,7 +110,8 @@ def initialize(cfg: ProgramConfig, make_executor: Callable[[ProgramConfig], obje
 
 def compute_numerical_solution(cfg: ProgramConfig, ex, integrator: str, b0: np.ndarray, u0: np.ndarray, rho0: np.ndarray,
                                num_steps: int, final_time: float, variable_u: bool = False, fused: bool = False,
-                               analytic: bool = True, save_integrals_over_time: bool = False) -> Dict[str, object]:
+                               analytic: bool = True, save_integrals_over_time: bool = False,
+                               divergence_cleaning=None) -> Dict[str, object]:
     """compute_numerical_solution.m: advance num_steps, then error / divergence / energy.
 
     save_integrals_over_time (I_RunOps('save_integrals_over_time'), compute_numerical_solution.m:15-19,
@@ -114,25 +119,42 @@ def compute_numerical_solution(cfg: ProgramConfig, ex, integrator: str, b0: np.n
     (calc_div + norm2_S) and the error against analytical_b at the current time (written into the
     second field register, as the reference does, and compared with norm2_diff); the final values
     complete the series, returned as energy_over_time, L2error_B_over_time, L2error_divB_over_time
-    (num_steps + 1 entries) and time = linspace(0, final_time, num_steps + 1)."""
+    (num_steps + 1 entries) and time = linspace(0, final_time, num_steps + 1).
+
+    divergence_cleaning = (absolute_error_threshold, max_iterations) is I_DC('divergence_cleaning')
+    (compute_numerical_solution.m:37-68,121-...): after every step the field is cleaned by clean_div
+    (+induction_eq/clean_div.m:3-81: CG on the program's -Laplacian to the absolute threshold, then the
+    projection), through the executor's own clean_div when it has one (the library's ieq_clean_div_cg),
+    else through the reference kernel sequence of clean_div_cg below.  Requires a program with the
+    divergence-cleaning module (cfg.laplace).  The number of CG iterations of every step is returned as
+    cg_iterations."""
     names, nf = rk_schedule(integrator, cfg.hall, variable_u)
     fields = [ex.dev_vec(b0)] + [ex.vec() for _ in range(nf - 1)]
     J, u, rho, tb = ex.vec(), ex.dev_vec(u0), ex.dev_scal(rho0), ex.time(0.0)
     args = fields + [J, u, rho, tb]
     series = {"energy": [], "L2error_B": [], "L2error_divB": []}
+    cg_iterations = []
     t0 = _time.perf_counter()
-    if save_integrals_over_time:
+    if save_integrals_over_time or divergence_cleaning is not None:
         div_t = ex.scal()
         for _ in range(num_steps):
-            series["energy"].append(ex.reduce("norm2", fields[0]))
-            ex.run("calc_div", [fields[0], div_t])
-            series["L2error_divB"].append(math.sqrt(max(ex.reduce("norm2_S", div_t), 0.0)))
-            ex.run("analytical_b", [fields[1], tb])
-            series["L2error_B"].append(math.sqrt(max(ex.reduce("norm2_diff", fields[0], fields[1]), 0.0)))
+            if save_integrals_over_time:
+                series["energy"].append(ex.reduce("norm2", fields[0]))
+                ex.run("calc_div", [fields[0], div_t])
+                series["L2error_divB"].append(math.sqrt(max(ex.reduce("norm2_S", div_t), 0.0)))
+                ex.run("analytical_b", [fields[1], tb])
+                series["L2error_B"].append(math.sqrt(max(ex.reduce("norm2_diff", fields[0], fields[1]), 0.0)))
             if fused:
                 ex.step(integrator, 1, args, variable_u=variable_u)
             else:
                 ex.run(names, args)
+            if divergence_cleaning is not None:
+                thr, max_it = divergence_cleaning
+                if hasattr(ex, "clean_div"):
+                    r = ex.clean_div(fields[0], thr, max_it)
+                else:
+                    r = clean_div_cg(cfg, ex, fields[0], thr, max_it)
+                cg_iterations.append(r["iterations"])
     elif fused:
         ex.step(integrator, num_steps, args, variable_u=variable_u)
     else:
@@ -146,6 +168,8 @@ def compute_numerical_solution(cfg: ProgramConfig, ex, integrator: str, b0: np.n
         ex.sync()
     runtime = _time.perf_counter() - t0
     res = {"runtime": runtime, "field_b": ex.host_vec(fields[0]), "time": ex.host_scal(tb)[:2] if hasattr(ex, "host_scal") else None}
+    if divergence_cleaning is not None:
+        res["cg_iterations"] = cg_iterations
     if analytic:
         ana, tt = ex.vec(), ex.time(final_time)
         ex.run("analytical_b", [ana, tt])

@@ -17,6 +17,9 @@ from typing import Dict, Iterable, Sequence
 import numpy as np
 
 TABLE_HEADER = "# N, order, form_uiBj, form_source, form_ujBi, runtime, energy, error in B, error in div B \n"
+# paper/divcleaning_confined_domain.m:158 (and divcleaning_hall_outflow.m:134)
+DC_TABLE_HEADER = ("# N, order, form_uiBj, form_source, form_ujBi, runtime, energy, error in B, error in div B, "
+                   "divcleaning, form_divcleaning\n")
 
 
 def table_row(n: int, order: int, forms: Sequence[str], res: Dict[str, object]) -> str:
@@ -25,10 +28,25 @@ def table_row(n: int, order: int, forms: Sequence[str], res: Dict[str, object]) 
         n, order, forms[0], forms[1], forms[2], res["runtime"], res["energy"], res["abs_err"], res["divergence_norm"])
 
 
-def write_convergence_table(path: str, rows: Iterable[str], device: str) -> None:
-    """Header, device line and the rows produced by table_row."""
+def dc_table_row(n: int, order: int, forms: Sequence[str], res: Dict[str, object], dc_form: str = "") -> str:
+    """One line of a divergence-cleaning sweep table (divcleaning_confined_domain.m:192-194): the table_row fields,
+    then 1 and the USE_LAPLACE_* define when the run cleaned every step, 0 and a blank otherwise."""
+    return "%d, %d, %s, %s, %s, %.15e, %.15e, %.15e, %.15e, %d, %s \n" % (
+        n, order, forms[0], forms[1], forms[2], res["runtime"], res["energy"], res.get("abs_err", 0.0),
+        res["divergence_norm"], 1 if dc_form else 0, dc_form if dc_form else " ")
+
+
+def dc_series_name(prefix: str, n: int, order: int, forms: Sequence[str], dc_label: str = "") -> str:
+    """File name of a run's time series (divcleaning_confined_domain.m:161-173): forms lowercased without the
+    USE_UIBJ_ / USE_SOURCE_ / USE_UJBI_ prefix, __DC_1_<WS_LN|WS_D0|NS_D0> or __DC_0."""
+    base = "%s_N_%d_order_%d_%s_%s_%s" % (prefix, n, order, forms[0][9:].lower(), forms[1][11:].lower(), forms[2][9:].lower())
+    return base + ("__DC_1_%s.h5" % dc_label if dc_label else "__DC_0.h5")
+
+
+def write_convergence_table(path: str, rows: Iterable[str], device: str, header: str = TABLE_HEADER) -> None:
+    """Header, device line and the rows produced by table_row (or dc_table_row with DC_TABLE_HEADER)."""
     with open(path, "w") as f:
-        f.write(TABLE_HEADER)
+        f.write(header)
         f.write("#  %s \n" % device)
         for r in rows:
             f.write(r)

@@ -180,6 +180,12 @@ class Program:
         _capi.check(rc, self.handle)
         return sec.value if timed else None
 
+    def diagnostics(self, b, time: float = 0.0) -> dict:
+        """FP64 energy, divergence norm and error norm of b (ieq_diagnostics), whatever REAL is."""
+        out = (ctypes.c_double * 3)()
+        _capi.check(self.lib.ieq_diagnostics(self.handle, ctypes.c_void_p(b.data_ptr()), float(time), out), self.handle)
+        return {"energy": out[0], "divergence_norm": out[1], "error_norm": out[2]}
+
     @property
     def last_step_fused(self) -> bool:
         """True when the last run / step call ran the fused stage kernels (ieq_last_step_path)."""

@@ -72,17 +72,28 @@ def _ensemble(ops, hall, diss, integrator, real, seeds):
     return cfgs, ens, Bs
 
 
+@pytest.mark.parametrize("shape", ["auto", "1", "32"])
 @pytest.mark.parametrize("integrator", ["SSPRK33", "CarpenterKennedy2N54", "ToulorgeDesmet2N84F"])
 @pytest.mark.parametrize("ops,hall,diss", [("6thOrder", True, "HIGH_ORDER"), ("4thOrder", False, "HIGH_ORDER"),
                                            ("2ndOrder", True, None)])
-def test_ensemble_members_match_oracle(integrator, ops, hall, diss):
+def test_ensemble_members_match_oracle(integrator, ops, hall, diss, shape, monkeypatch):
+    """Tile shape chosen by the tuner, or forced (IEQ_ENS_SHAPE): 1 the one-kernel Hall stage (E formed in the
+    stage kernel), 32 the EMF kernel + stage kernel pair."""
     import torch
+
+    if shape != "auto":
+        monkeypatch.setenv("IEQ_ENS_SHAPE", shape)
 
     cfgs, ens, Bs = _ensemble(ops, hall, diss, integrator, "double", seeds=(1, 2, 3))
     nsteps = 2
     ens.step(nsteps)
     torch.cuda.synchronize()
-    assert ens.config()["tile"] in ("64x4", "64x8", "32x8")
+    tile = ens.config()["tile"]
+    assert tile in ("64x4", "64x8", "32x8", "hall")
+    if shape == "1":
+        assert tile == ("hall" if hall else "64x8")
+    elif shape == "32":
+        assert tile == "32x8"
     for i, (cfg, B) in enumerate(zip(cfgs, Bs)):
         want = _oracle_steps(cfg, integrator, B, MEMBERS[i][3], nsteps)
         got = ens.state(i).numpy()

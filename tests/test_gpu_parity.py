@@ -226,3 +226,36 @@ def test_fused_step_matches_reference_schedule(ops, integrator, hall, diss, u, r
     _cmp_vec(prog, db[0], hb[0], cfg, tol, "b1 after %d steps" % nsteps)
     ttol = 1e-15 if real == "double" else 1e-7
     assert abs(float(dt_[0]) - float(ht[0])) <= ttol and abs(float(dt_[1]) - float(ht[1])) <= ttol
+
+
+@pytest.mark.parametrize("real", ["double", "float"])
+@pytest.mark.parametrize("tc", ["rotation_2D", "rotation_3D", "hall_periodic", "hall_travelling_wave", "confined_domain",
+                                "alfven_periodic_2D", "gaussian_rotation_3D", "uniform_flow"])
+def test_init_and_analytic_kernels_all_testcases(tc, real):
+    """kernel_init.cl (init_b, init_u, init_rho, analytical_b at t = 0.37) for every testcase vs the oracle."""
+    periodic = tc in ("hall_periodic", "hall_travelling_wave", "alfven_periodic_2D", "uniform_flow")
+    lo, hi = (0.0, 2 * math.pi) if periodic else (-1.0, 1.0)
+    cfg = ProgramConfig(real=real, opset="4thOrder", nx=13, ny=14, nz=15, xmin=lo, xmax=hi, ymin=lo, ymax=hi, zmin=lo,
+                        zmax=hi, periodic=periodic, testcase=tc, ux=0.3, uy=-0.2, uz=0.1, w_size=64)
+    prog, orc = _pair(cfg)
+    tol = 1e-12 if real == "double" else 1e-5
+    import torch
+
+    for name, make_h, make_d in (("init_b", orc.vec, prog.vector), ("init_u", orc.vec, prog.vector),
+                                 ("init_rho", orc.scal, prog.scalar)):
+        h, d = make_h(), make_d()
+        orc.run(name, [h])
+        prog.run(name, [d])
+        torch.cuda.synchronize()
+        if name == "init_rho":
+            assert _relerr(prog.download_real(d)[: cfg.num_nodes], h[: cfg.num_nodes]) <= tol, name
+        elif np.abs(h).max() > 0:
+            _cmp_vec(prog, d, h, cfg, tol, name)
+        else:
+            assert np.abs(prog.download_real4(d)).max() == 0, name
+    h, d = orc.vec(), prog.vector()
+    orc.run("analytical_b", [h, orc.time(0.37)])
+    prog.run("analytical_b", [d, prog.time_buffer(0.37)])
+    torch.cuda.synchronize()
+    if np.abs(h).max() > 0:
+        _cmp_vec(prog, d, h, cfg, tol, "analytical_b")

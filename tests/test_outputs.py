@@ -6,6 +6,7 @@ the sweep tables follow paper/convergence_hall_periodic.m:119-146, the time seri
 examples/energy_hall_outflow.m:150-161.
 """
 import math
+import os
 
 import numpy as np
 
@@ -88,3 +89,33 @@ def test_convergence_sweep_table(tmp_path):
     rates = observed_orders(runs)
     for (order, _), (rate,) in rates.items():
         assert rate > order - 0.2, (order, rate)
+
+
+def test_divcleaning_sweep_table_and_series(tmp_path):
+    """divcleaning_sweep (paper/divcleaning_confined_domain.m:130-212) on the oracle: per order and form triple one
+    cleaned run per -Laplacian and one uncleaned run, each with its time series under the reference's file name
+    and one table line in the 11-column format.  The confined_domain field is discretely divergence free to
+    round-off, so no CG iteration is needed (r.r below the threshold) and the cleaned runs equal the uncleaned
+    one."""
+    from pyoracle import Oracle
+
+    from inductioneq_amd.outputs import DC_TABLE_HEADER
+    from inductioneq_amd.sweeps import FORMS_PAPER, divcleaning_sweep
+
+    path = tmp_path / "confined_domain_divcleaning.txt"
+    runs = divcleaning_sweep(10, (2,), [FORMS_PAPER[3]], final_time=0.1, make_executor=Oracle, path=str(path),
+                             series_dir=str(tmp_path))
+    assert [r["dc_form"] for r in runs] == ["WS_LN", "WS_D0", "NS_D0", ""]
+    lines = open(path).read().splitlines()
+    assert lines[0] + "\n" == DC_TABLE_HEADER and len(lines) == 2 + 4
+    assert lines[2].split(", ")[9:] == ["1", "USE_LAPLACE_WIDE_STENCIL_LNS "]
+    assert lines[5].split(", ")[9] == "0"
+    names = sorted(os.path.basename(r["series"]) for r in runs)
+    assert "confined_domain_N_10_order_2_product_central_product__DC_1_WS_LN" in names[0] or any(
+        n.startswith("confined_domain_N_10_order_2_product_central_product__DC_1_WS_LN") for n in names)
+    assert any(n.startswith("confined_domain_N_10_order_2_product_central_product__DC_0") for n in names)
+    plain = runs[-1]["result"]
+    for r in runs[:-1]:
+        assert max(r["result"]["cg_iterations"]) == 0
+        assert np.array_equal(r["result"]["field_b"], plain["field_b"])
+    assert all(len(r["result"]["energy_over_time"]) == len(r["result"]["time_series"]) for r in runs)
