@@ -170,6 +170,15 @@ int ieq_last_step_path(const ieq_program *prog);
  * are added in a fixed order (deterministic).  Waits for the result. */
 int ieq_diagnostics(ieq_program *prog, const void *b, double time, double out[3]);
 
+/* ieq_diagnostics for callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  No counterpart in the reference.  Same
+ * kernels and results as ieq_diagnostics, but nothing is allocated and nothing waits: the three
+ * results go to out_dev (3 doubles of device memory), enqueued on the program's stream.  The
+ * partial sums live in a static device array of 1024 x 3 doubles (24 KiB per GPU, held until the
+ * process ends); the partial pass uses at most 1024 blocks whatever the grid size, so every grid
+ * fits, and every call overwrites the entries it reads.  Calls that overlap on one GPU (several
+ * streams or host threads at once) race on that array. */
+int ieq_diagnostics_noalloc(ieq_program *prog, const void *b, double time, double *out_dev);
 
 /* Result of ieq_clean_div_cg. */
 typedef struct {

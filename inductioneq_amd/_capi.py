@@ -22,7 +22,7 @@ EXPORTS = [
     "ieq_program_build", "ieq_program_release", "ieq_last_error", "ieq_program_get_info", "ieq_set_stream",
     "ieq_synchronize", "ieq_alloc", "ieq_free", "ieq_vector_field_bytes", "ieq_scalar_field_bytes",
     "ieq_upload_real4", "ieq_download_real4", "ieq_upload_real", "ieq_download_real", "ieq_run", "ieq_run_host",
-    "ieq_step", "ieq_step_uniform", "ieq_last_step_path", "ieq_diagnostics",
+    "ieq_step", "ieq_step_uniform", "ieq_last_step_path", "ieq_diagnostics", "ieq_diagnostics_noalloc",
     "ieq_clean_div_cg", "ieq_member_stride", "ieq_ensemble_create", "ieq_ensemble_step", "ieq_ensemble_config",
     "ieq_ensemble_release", "ieq_adjoint_create", "ieq_adjoint_forward", "ieq_adjoint_backward", "ieq_adjoint_counts",
     "ieq_adjoint_release",
@@ -148,6 +148,7 @@ def load() -> ctypes.CDLL:
                                      ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     lib.ieq_last_step_path.argtypes = [P]
     lib.ieq_diagnostics.argtypes = [P, P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+    lib.ieq_diagnostics_noalloc.argtypes = [P, P, ctypes.c_double, P]
     lib.ieq_clean_div_cg.argtypes = [P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(CgResult)]
     lib.ieq_member_stride.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.ieq_member_stride.restype = ctypes.c_longlong

@@ -758,6 +758,20 @@ int ieq_diagnostics(ieq_program* prog, const void* b, double time, double out[3]
   return IEQ_OK;
 }
 
+int ieq_diagnostics_noalloc(ieq_program* prog, const void* b, double time, double* out_dev) {
+  if (!prog || !b || !out_dev) return IEQ_ERR_ARGS;
+  if (ieq::diag_blocks(prog->p) > ieq::kDiagBlocks)  // the partial pass never uses more blocks than the scratch holds
+    return prog->fail(IEQ_ERR_ARGS, "diagnostics need more partial sums than the static scratch holds");
+  double* part = nullptr;
+  hipError_t e = ieq::diag_scratch(&part);
+  if (e != hipSuccess) return hip_fail(prog, e, "diagnostics scratch lookup");
+  const int analytic = prog->info.testcase != 7;
+  e = prog->info.real_bytes == 8
+          ? ieq::diag_launch<double>(prog->p, prog->info.opset, prog->stream, (const double*)b, time, analytic, part, out_dev)
+          : ieq::diag_launch<float>(prog->p, prog->info.opset, prog->stream, (const float*)b, time, analytic, part, out_dev);
+  if (e != hipSuccess) return hip_fail(prog, e, "diagnostics launch");
+  return IEQ_OK;
+}
 
 }  // extern "C"
 

@@ -5,12 +5,22 @@
 // one pass with every product and sum in double.  A fixed number of blocks strides over the
 // nodes; each block writes one partial triple and a second one-block kernel adds the
 // partials in a fixed order, so the result is deterministic and independent of the caller.
+#include <mutex>
+
 #include "ieq_device.hpp"
 #include "ieq_diag.hpp"
 
 namespace ieq {
 namespace {
 
+// Partial sums of ieq_diagnostics_noalloc: kDiagBlocks triples (24 KiB), one copy per GPU, held
+// until the process ends.  Host code never passes this name to a kernel or a copy: it uses the
+// device address that diag_scratch() looked up for the current device.
+__device__ __align__(256) double scratch[3 * kDiagBlocks];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
 
 template <int OPS>
 __device__ inline double ddiv_dir(const Params& p, const double* hd, int d, uint32_t ix, uint32_t iy, uint32_t iz,
@@ -94,6 +104,22 @@ hipError_t diag_launch(const Params& p, int ops, hipStream_t s, const R* b, doub
   }
   k_diag_sum<<<1, 64, 0, s>>>(part, nblk, analytic, out_dev);
   return hipGetLastError();
+}
+
+hipError_t diag_scratch(double** ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return hipSuccess;
 }
 
 template hipError_t diag_launch<double>(const Params&, int, hipStream_t, const double*, double, int, double*, double*);

@@ -13,6 +13,10 @@ constexpr int kDiagBlocks = 1024, kDiagThreads = 256;
 // holds 3 doubles per block.
 int diag_blocks(const Params& p);
 
+// Device address, on the current device, of the library's static partial buffer (3 * kDiagBlocks
+// doubles) used by ieq_diagnostics_noalloc; looked up once per device.
+hipError_t diag_scratch(double** ptr);
+
 // Enqueue the partial pass into `part` (3 * diag_blocks(p) doubles) and the fixed-order sum into
 // out_dev[0..2] = (energy, divergence norm, error norm); `analytic` selects the error term
 // against b_analytical at time t.
