@@ -92,14 +92,17 @@ def traffic_summary(fetch_csv, write_csv, copyf_csv, copyw_csv, copy_mib=512):
     return {"fetch_factor": ff, "write_factor": wf, "kernels": kern}
 
 
-def stored_traffic(name):
-    """Traffic per RK stage (k_emf + k_stage dispatches) from a committed summary, or None."""
+def stored_traffic(name, real):
+    """Traffic per RK stage of REAL type `real` (the k_hall_stage dispatch, or k_emf + k_stage of the two-kernel
+    form) from a committed summary, or None."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     d = json.load(open(path))
-    return {"bytes_per_stage": sum(v["bytes"] for k, v in d["kernels"].items() if k.startswith(("k_emf", "k_stage"))),
-            "source": "profiles/" + name}
+    kern = {k: v for k, v in d["kernels"].items() if ("<%s," % real) in k}
+    one = [v["bytes"] for k, v in kern.items() if k.startswith("k_hall_stage")]
+    total = sum(one) if one else sum(v["bytes"] for k, v in kern.items() if k.startswith(("k_emf", "k_stage")))
+    return {"bytes_per_stage": total, "source": "profiles/" + name} if total else None
 
 
 def main():
@@ -153,7 +156,7 @@ def main():
         cpu = cpu_baseline(cfg, B0, a.integrator, a.cpu_steps, stages)
     traffic = None
     if n == 128 and a.integrator == "CarpenterKennedy2N54":
-        traffic = stored_traffic("hall128_%s_traffic.json" % a.precision)
+        traffic = stored_traffic("hall128_%s_traffic.json" % a.precision, real)
 
     out = {
         "metric": "Mcell-updates/s per RK stage",
@@ -383,7 +386,8 @@ def run_ensemble(a, steps):
                    "cells": ncell, "sizes": [int(v) for v in sizes], "launch": cfg_e},
         "roofline": {"bound": "hbm", "achieved": round(alg / (sec32 / stages) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(alg / (sec32 / stages) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "one batched RK stage (k_emf_batched + k_stage_batched), 32 B/cell algorithmic"},
+                     "kernel": "one batched RK stage (%s), 32 B/cell algorithmic"
+                               % ("k_hall_stage_batched" if cfg_e["tile"] == "hall" else "k_emf_batched + k_stage_batched")},
         "per_instance_fp64": {"seconds": round(sec64, 4), "value": round(ncell * stages / sec64 / 1e6, 3)},
         "speedup_vs_per_instance_fp64": round(sec64 / sec32, 3),
         "max_rel_err_vs_fp64": err,
@@ -586,7 +590,13 @@ def bench_rotation(a):
         "rel_err_vs_analytic": res["rel_err"],
         "divergence_norm": res["divergence_norm"],
         "oracle_parity": parity,
-        "roofline": None,
+        # 88 B/cell: the SSPRK33 stages read b1, u / b1, b2, u / b1, b3, u and write one field each (average over
+        # the three stages, FP64); at 40^3 the 6 MB working set is cache-resident, so frac is not an HBM figure
+        "roofline": {"bound": "hbm", "achieved": round(88 * n ** 3 * stages / sec / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(88 * n ** 3 * stages / sec / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "reference-schedule kernels (one launch per SSPRK33 stage), 88 B/cell "
+                                                "algorithmic", "us_per_stage": round(sec * 1e6 / stages, 2),
+                     "cache_resident": 88 * n ** 3 < 256e6},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
