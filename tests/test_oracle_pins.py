@@ -12,8 +12,10 @@
 * Adaptive dissipation (artificial_dissipation.h:12-45, artificial_dissipation.cl:118-160):
   continuity at both branch points of kappa, the k^2 scaling of the thresholds with the
   stencil offset, monotonicity in the jump, and invariance under the joint scaling
-  dx -> lambda dx, MP2MIN/MP2MAX -> MP2MIN/MP2MAX / lambda.  The shape of the ramp between
-  the branch points (a square root) is restated, not pinned.
+  dx -> lambda dx, MP2MIN/MP2MAX -> MP2MIN/MP2MAX / lambda; the ramp between the branch
+  points is the square root of the linear interpolation between them, the closed form of the
+  reference's documentation (docs/artificial_dissipation.ipynb), recovered from the oracle's
+  dissipation at a single spike.
 * The reference testcases' analytic solutions (include/rotation_3D.h, confined_domain.h,
   hall_travelling_wave.h, alfven_periodic_2D.h, and the Gaussian rotation of the rotation
   benchmark): the oracle's time loop, as in examples/compute_error_*.m, converges to each at
@@ -215,6 +217,30 @@ def test_adaptive_dissipation_thresholds_scale_with_the_offset_squared():
     v4, _ = _adaptive_at("4thOrder", 16, d4, U, k_node=2)
     v4_small, _ = _adaptive_at("4thOrder", 16, d4 * 1e-3, U, k_node=2)
     assert abs(v4 - 1e3 * v4_small) > 1e-6 * abs(v4)
+
+
+@pytest.mark.parametrize("ops", ["2ndOrder", "4thOrder"])
+def test_adaptive_dissipation_ramp_is_the_square_root_of_the_linear_interpolation(ops):
+    """Between the branch points kappa_k(s) = sqrt((s - lo) / (hi - lo)), lo = k^2 MP2MIN CMIN dx and
+    hi = k^2 MP2MAX CMAX dx (docs/artificial_dissipation.ipynb, the kappa_k cases and "the term whose square root is
+    used is a linear interpolation between zero and one"; Svard and Mishra 2009, eq. 40).  At the node next to a spike
+    of height delta only the spike pair has a jump, so the dissipation per unit jump is
+    beta + kappa(s) (alpha - beta) with s = U delta^2: beta is its value below lo (kappa = 0), alpha above hi
+    (kappa = 1).  kappa recovered from the oracle must square to the linear interpolation; a linear ramp (no square
+    root) would give kappa^2 = f^2 instead of f."""
+    U = 1.0
+    _, dx = _adaptive_at(ops, 16, 1.0, U)
+    lo, hi = 0.5 * dx, 4.0 * dx  # k = 1, MP2MIN 0.5, MP2MAX 4.0, CMIN = CMAX = 1
+
+    def per_jump(s):
+        d = math.sqrt(s / U)
+        return _adaptive_at(ops, 16, d, U)[0] / d
+
+    beta, alpha = per_jump(0.5 * lo), per_jump(2.0 * hi)
+    assert abs(alpha - beta) > 1e-3 * abs(beta)
+    for f in (0.1, 0.25, 0.5, 0.81, 0.95):
+        kappa = (per_jump(lo + f * (hi - lo)) - beta) / (alpha - beta)
+        assert abs(kappa * kappa - f) <= 1e-9, (f, kappa)
 
 
 def test_adaptive_dissipation_invariant_under_joint_dx_threshold_scaling():
