@@ -156,8 +156,11 @@ int ieq_step_uniform(ieq_program *prog, const char *integrator, long long nsteps
                      void *const *args, int nargs, double *kernel_seconds);
 
 /* Path taken by the last successful ieq_run / ieq_step / ieq_step_uniform on prog: 1 the fused
- * stage kernels, 0 the reference kernel sequence, -1 none (or the call failed).  Setting
- * IEQ_NO_FUSED=1 in the environment keeps every call on the reference kernel sequence. */
+ * stage kernels, 2 the reference kernel sequence with its stage kernels run as z-marching tiles
+ * (whole-grid NDRange, no Hall term, no adaptive dissipation), 0 the reference kernel sequence
+ * with one work-item per node, -1 none (or the call failed).  Setting IEQ_NO_FUSED=1 in the
+ * environment keeps every call on the reference kernel sequence, IEQ_NO_TILED=1 keeps its stage
+ * kernels at one work-item per node. */
 int ieq_last_step_path(const ieq_program *prog);
 
 /* FP64 diagnostics of the vector field b at time `time`, whatever REAL is, in one pass with every

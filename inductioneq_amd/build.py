@@ -21,7 +21,7 @@ ARCH = os.environ.get("IEQ_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["--offload-arch=%s" % ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["ieq_fused.hip", "ieq_ensemble.hip", "ieq_hall_f64.hip", "ieq_hall_f32.hip", "ieq_generic.hip", "ieq_adjoint.hip",
+SOURCES = ["ieq_fused.hip", "ieq_ensemble.hip", "ieq_hall_f64.hip", "ieq_hall_f32.hip", "ieq_tiled_f64.hip", "ieq_tiled_f32.hip", "ieq_generic.hip", "ieq_adjoint.hip",
            "ieq_cg.hip", "ieq_diag.hip", "ieq_capi.hip", "ieq_kernel_exports.hip"]
 
 

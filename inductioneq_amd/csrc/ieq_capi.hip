@@ -18,6 +18,7 @@
 #include "ieq_diag.hpp"
 #include "ieq_fused.hpp"
 #include "ieq_generic.hpp"
+#include "ieq_tiled.hpp"
 
 using ieq::Params;
 
@@ -170,7 +171,9 @@ struct ieq_program {
   std::string err;
   ieq::FusedWork* fused = nullptr;
   ieq::CgWork* cg = nullptr;
-  int last_path = -1;  // path of the last ieq_run / ieq_step: 0 reference kernels, 1 fused kernels
+  int last_path = -1;  // path of the last ieq_run / ieq_step: 0 reference kernels, 1 fused kernels, 2 reference
+                       // sequence with the tiled stage kernel
+  bool tiled_used = false;  // the current ieq_run / ieq_step launched a tiled stage
   double* diag = nullptr;  // ieq_diagnostics: 3 * kDiagBlocks partials + the 3 results
 
   bool has(const char* m) const { return modules.count(m) != 0; }
@@ -358,6 +361,11 @@ bool range_ok(const ieq_program* pr, const uint32_t g[3], int off) {
          g[2] + (uint32_t)off <= (uint32_t)pr->info.nz;
 }
 
+// Does the NDRange g cover the program's whole grid?
+bool full_grid(const ieq_program* pr, const uint32_t g[3]) {
+  return g[0] == (uint32_t)pr->info.nx && g[1] == (uint32_t)pr->info.ny && g[2] == (uint32_t)pr->info.nz;
+}
+
 template <typename R>
 int run_one(ieq_program* pr, const char* name, const uint32_t g[3], const uint32_t l[3], void* const* args, int nargs) {
   const Params& p = pr->p;
@@ -377,10 +385,20 @@ int run_one(ieq_program* pr, const char* name, const uint32_t g[3], const uint32
     R* F[3] = {A(0), A(1), st->nf == 3 ? A(2) : nullptr};
     R *J = A(st->nf), *u = A(st->nf + 1), *rho = A(st->nf + 2), *time = A(st->nf + 3);
     switch (st->kind) {
-      case K_STAGE:
-        ieq::gen_stage<R>(p, ops, s, g, F[st->out], st->P >= 0 ? F[st->P] : nullptr, (R)st->cP, st->Q >= 0 ? F[st->Q] : nullptr,
-                          (R)st->cQ, F[st->Y], (R)(st->cD * p.dt), J, u, time);
+      case K_STAGE: {
+        R* P = st->P >= 0 ? F[st->P] : nullptr;
+        R* Q = st->Q >= 0 ? F[st->Q] : nullptr;
+        // the whole grid of a program the tiled kernel covers: z-marching tiles; otherwise (or for
+        // a part of the grid) one work-item per node
+        if (full_grid(pr, g) && ieq::tiled_stage_ok(p, ops, (int)sizeof(R))) {
+          hipError_t e = ieq::tiled_stage<R>(p, ops, s, F[st->out], P, (R)st->cP, Q, (R)st->cQ, F[st->Y], (R)(st->cD * p.dt), u, time);
+          if (e != hipSuccess) return hip_fail(pr, e, name);
+          pr->tiled_used = true;
+        } else {
+          ieq::gen_stage<R>(p, ops, s, g, F[st->out], P, (R)st->cP, Q, (R)st->cQ, F[st->Y], (R)(st->cD * p.dt), J, u, time);
+        }
         break;
+      }
       case K_2R:
         ieq::gen_update_2r<R>(p, s, g, F[st->P], F[st->out], F[st->Q], (R)st->cP, (R)st->cQ, time, st->tc);
         break;
@@ -509,9 +527,6 @@ bool fused_allowed() {
   return !(v && *v && std::strcmp(v, "0") != 0);
 }
 
-bool full_grid(const ieq_program* pr, const uint32_t g[3]) {
-  return g[0] == (uint32_t)pr->info.nx && g[1] == (uint32_t)pr->info.ny && g[2] == (uint32_t)pr->info.nz;
-}
 
 // Is names[0..n) the RK_Step list of one integrator (compute_numerical_solution_setup.m:140-371, without
 // calc_u) repeated k >= 1 times, with that integrator's argument count?
@@ -664,8 +679,9 @@ int ieq_run(ieq_program* prog, const char* const* names, int n, const uint32_t g
     opt.final_curl = true;  // every argument ends as after the listed kernels
     rc = ieq::fused_step(prog->p, prog->info, prog->stream, &prog->fused, integ, k, 0, args, opt, &used, prog->err);
   }
+  prog->tiled_used = false;
   if (rc == IEQ_OK && !used) rc = run_list(prog, names, n, g, l, args, nargs);
-  prog->last_path = rc == IEQ_OK ? used : -1;
+  prog->last_path = rc == IEQ_OK ? (used ? 1 : prog->tiled_used ? 2 : 0) : -1;
   tm.end(prog, rc, kernel_seconds);
   return rc;
 }
@@ -714,6 +730,7 @@ static int step_impl(ieq_program* prog, const char* integrator, long long nsteps
     rc = ieq::fused_step(prog->p, prog->info, prog->stream, &prog->fused, integrator, nsteps, variable_u, args, opt,
                          &used_fused, prog->err);
   }
+  prog->tiled_used = false;
   if (rc == IEQ_OK && !used_fused) {
     std::vector<const char*> names;
     for (auto& s : sched) names.push_back(s.c_str());
@@ -721,7 +738,7 @@ static int step_impl(ieq_program* prog, const char* integrator, long long nsteps
     uint32_t l[3] = {(uint32_t)prog->info.w_size, 1, 1};
     for (long long k = 0; k < nsteps && rc == IEQ_OK; ++k) rc = run_list(prog, names.data(), (int)names.size(), g, l, args, nargs);
   }
-  prog->last_path = rc == IEQ_OK ? used_fused : -1;
+  prog->last_path = rc == IEQ_OK ? (used_fused ? 1 : prog->tiled_used ? 2 : 0) : -1;
   tm.end(prog, rc, kernel_seconds);
   return rc;
 }

@@ -89,6 +89,38 @@ template <typename R> __device__ inline void st3(R* __restrict__ f, long long s,
 }
 
 // ---------------------------------------------------------------------------------
+// Two-point fluxes shared by the reference-schedule and the tiled stage kernels.
+template <typename R> __device__ inline R cmp(V3<R> v, int d) { return d == 0 ? v.x : d == 1 ? v.y : v.z; }
+
+// Two-point volume flux of the linear term along direction d
+// (induction_Eq_volume.cl:13-165): f = f(u_i B_j) - f(u_i d_j B_j) - f(u_j B_i).
+template <typename R>
+__device__ inline V3<R> volume_flux(const Params& p, int d, V3<R> Bm, V3<R> um, V3<R> Bk, V3<R> uk) {
+  R bmd = cmp(Bm, d), bkd = cmp(Bk, d), umd = cmp(um, d), ukd = cmp(uk, d);
+  V3<R> f1, f2, f3;
+  switch (p.form_uibj) {
+    case 0: f1 = R(0.5) * (bmd * um + bkd * uk); break;
+    case 1: f1 = (R(0.25) * (bmd + bkd)) * (um + uk); break;
+    default: f1 = R(0.5) * (bkd * um + bmd * uk); break;
+  }
+  switch (p.form_source) {
+    case 0: f2 = zero3<R>(); break;
+    case 1: f2 = (R(0.5) * (bkd - bmd)) * um; break;
+    default: f2 = (R(0.25) * (bkd - bmd)) * (um + uk); break;
+  }
+  switch (p.form_ujbi) {
+    case 0: f3 = R(0.5) * (umd * Bm + ukd * Bk); break;
+    case 1: f3 = (R(0.25) * (umd + ukd)) * (Bm + Bk); break;
+    default: f3 = R(0.5) * (umd * Bk + ukd * Bm); break;
+  }
+  return (f1 - f2) - f3;
+}
+
+template <typename R> __device__ inline V3<R> sigma3(V3<R> a, V3<R> b) {
+  return {fmax(fabs(a.x), fabs(b.x)), fmax(fabs(a.y), fabs(b.y)), fmax(fabs(a.z), fabs(b.z))};
+}
+
+// ---------------------------------------------------------------------------------
 // Boundary row selection: row of a (2nb+1)-row table for node i of n nodes.  Equals
 // the reference's accumulation over check_bound_l / check_bound_*r (utils.h:172-201),
 // including its behaviour on grids shorter than 2*nb.

@@ -10,32 +10,6 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 
-template <typename R> __device__ inline R cmp(V3<R> v, int d) { return d == 0 ? v.x : d == 1 ? v.y : v.z; }
-
-// Two-point volume flux of the linear term along direction d
-// (induction_Eq_volume.cl:13-165): f = f(u_i B_j) - f(u_i d_j B_j) - f(u_j B_i).
-template <typename R>
-__device__ inline V3<R> volume_flux(const Params& p, int d, V3<R> Bm, V3<R> um, V3<R> Bk, V3<R> uk) {
-  R bmd = cmp(Bm, d), bkd = cmp(Bk, d), umd = cmp(um, d), ukd = cmp(uk, d);
-  V3<R> f1, f2, f3;
-  switch (p.form_uibj) {
-    case 0: f1 = R(0.5) * (bmd * um + bkd * uk); break;
-    case 1: f1 = (R(0.25) * (bmd + bkd)) * (um + uk); break;
-    default: f1 = R(0.5) * (bkd * um + bmd * uk); break;
-  }
-  switch (p.form_source) {
-    case 0: f2 = zero3<R>(); break;
-    case 1: f2 = (R(0.5) * (bkd - bmd)) * um; break;
-    default: f2 = (R(0.25) * (bkd - bmd)) * (um + uk); break;
-  }
-  switch (p.form_ujbi) {
-    case 0: f3 = R(0.5) * (umd * Bm + ukd * Bk); break;
-    case 1: f3 = (R(0.25) * (umd + ukd)) * (Bm + Bk); break;
-    default: f3 = R(0.5) * (umd * Bk + ukd * Bm); break;
-  }
-  return (f1 - f2) - f3;
-}
-
 // kappa4 of include/artificial_dissipation.h:23-45 (k enters only through k*k).
 template <typename R>
 __device__ inline V3<R> kappa3(const Params& p, uint32_t k, R h, V3<R> b1, V3<R> b2, V3<R> u1, V3<R> u2) {
@@ -49,10 +23,6 @@ __device__ inline V3<R> kappa3(const Params& p, uint32_t k, R h, V3<R> b1, V3<R>
     out[c] = (R)((lo < s) && (s < hi)) * sqrt(fabs(s - lo) / fabs(hi - lo)) + (R)(s >= hi);
   }
   return {out[0], out[1], out[2]};
-}
-
-template <typename R> __device__ inline V3<R> sigma3(V3<R> a, V3<R> b) {
-  return {fmax(fabs(a.x), fabs(b.x)), fmax(fabs(a.y), fabs(b.y)), fmax(fabs(a.z), fabs(b.z))};
 }
 
 // One HOD partial sum, diff_HOD_* of include/artificial_dissipation.h:58-101.

@@ -191,6 +191,12 @@ class Program:
         """True when the last run / step call ran the fused stage kernels (ieq_last_step_path)."""
         return self.lib.ieq_last_step_path(self.handle) == 1
 
+    @property
+    def last_step_path(self) -> str:
+        """Path of the last run / step call (ieq_last_step_path): "fused", "tiled" (the reference kernel
+        sequence with the z-marching stage kernel), "reference" (one work-item per node) or "none"."""
+        return {1: "fused", 2: "tiled", 0: "reference"}.get(self.lib.ieq_last_step_path(self.handle), "none")
+
     def reduce(self, name: str, *fields) -> float:
         """Run a group-reduction kernel and return the host sum of its partials."""
         out = self.reduce_output()
