@@ -237,7 +237,8 @@ int ieq_ensemble_create(ieq_ensemble **ens, ieq_program *prog, const ieq_member 
 int ieq_ensemble_step(ieq_ensemble *ens, long long nsteps, double *kernel_seconds);
 
 /* Chosen launch configuration: tile shape code (4: 64 x 4 cells, 8: 64 x 8, 32: 32 x 8 with the EMF
- * in its own launch; 1: the one-kernel Hall stage, 32 x 8 in FP64 and 64 x 8 in FP32),
+ * in its own launch; 1: the one-kernel Hall stage, 32 x 8 in FP64 and 64 x 8 in FP32; 2: the
+ * one-kernel Hall stage with 16 x 16 tiles in FP64 and 32 x 8 in FP32),
  * z-chunk, order (0 member order, 1 largest first; +2 when the step is replayed from a
  * hipGraph rather than launched directly, whichever timed faster). */
 int ieq_ensemble_config(const ieq_ensemble *ens, int *tile_shape, int *z_chunk, int *order);

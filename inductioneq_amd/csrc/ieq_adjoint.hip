@@ -289,6 +289,7 @@ template <typename R> FArgs<R> base_args(const Adjoint& A, const StencilCoef<R>&
     a.two_h[d] = (R)(2.0 / h[d]);
   }
   a.rho = (R)(1.0 / A.c_hall);
+  for (int d = 0; d < 3; ++d) a.jsc[d] = (R)(A.c_hall / h[d]);
   for (int d = 0; d < 3; ++d) a.u[d] = (R)A.u[d];
   a.conv = (A.u[0] != 0.0 || A.u[1] != 0.0 || A.u[2] != 0.0);
   a.src_zero = A.p.form_source == 0;

@@ -62,6 +62,7 @@ template <typename R> FArgs<R> member_base(const Ensemble& e, const ieq_member& 
     a.two_h[d] = (R)(2.0 / h[d]);
   }
   a.rho = (R)mb.rho;
+  for (int d = 0; d < 3; ++d) a.jsc[d] = (R)(1.0 / (h[d] * mb.rho));
   a.u[0] = (R)mb.ux;
   a.u[1] = (R)mb.uy;
   a.u[2] = (R)mb.uz;
@@ -118,23 +119,25 @@ template <typename R> std::vector<FArgs<R>> ensemble_args(Ensemble& e) {
 }
 
 // Tile shapes by code, as for the single-grid launches: 4 -> 64 x 4, 8 -> 64 x 8, 32 -> 32 x 8
-// (32-wide tiles waste fewer lanes on members whose nx is not a multiple of 64); code 1 is the
-// one-kernel Hall stage (k_hall_stage_batched, E formed in the stage kernel) with its own tile.
-constexpr int kHallCode = 1;
+// (32-wide tiles waste fewer lanes on members whose nx is not a multiple of 64); codes 1 and 2
+// are the one-kernel Hall stage (k_hall_stage_batched, E formed in the stage kernel) with its
+// single-grid tile (1) or its second, narrower tile (2: FP32 32 x 8, FP64 16 x 16).
+constexpr int kHallCode = 1, kHallCode2 = 2;
+bool is_hall_code(int code) { return code == kHallCode || code == kHallCode2; }
 int ens_tx(int code, int real_bytes) {
-  if (code == kHallCode) {
+  if (is_hall_code(code)) {
     int tx = 0, ty = 0;
-    if (real_bytes == 8) hall_tile_shape<double>(&tx, &ty);
-    else hall_tile_shape<float>(&tx, &ty);
+    if (real_bytes == 8) hall_tile_shape<double>(&tx, &ty, code == kHallCode2);
+    else hall_tile_shape<float>(&tx, &ty, code == kHallCode2);
     return tx;
   }
   return code == 32 ? 32 : 64;
 }
 int ens_ty(int code, int real_bytes) {
-  if (code == kHallCode) {
+  if (is_hall_code(code)) {
     int tx = 0, ty = 0;
-    if (real_bytes == 8) hall_tile_shape<double>(&tx, &ty);
-    else hall_tile_shape<float>(&tx, &ty);
+    if (real_bytes == 8) hall_tile_shape<double>(&tx, &ty, code == kHallCode2);
+    else hall_tile_shape<float>(&tx, &ty, code == kHallCode2);
     return ty;
   }
   return code == 32 ? 8 : code;
@@ -185,9 +188,10 @@ template <typename R, int RR, int TY, int TX> const void* ens_kernel_t(const Ens
 // Kernel of a plan entry (kind 3: EMF, otherwise the stage kernel) for the ensemble's settings;
 // null for the EMF entries under the one-kernel Hall stage.
 template <typename R> const void* ens_kernel(const Ensemble& e, int code, const StageSpec& st) {
-  if (code == kHallCode) {
+  if (is_hall_code(code)) {
     if (st.kind == 3) return nullptr;
-    return hall_stage_kernel<R>(e.rr, stage_mode(st.kind == 4, st.P, st.Q, st.Y), e.hod, e.conv != 0, true);
+    return hall_stage_kernel<R>(e.rr, stage_mode(st.kind == 4, st.P, st.Q, st.Y), e.hod, e.conv != 0, true,
+                                code == kHallCode2);
   }
   if (code == 32) {
     if (e.rr == 1) return ens_kernel_t<R, 1, 8, 32>(e, st);
@@ -325,14 +329,14 @@ template <typename R> int ens_tune(Ensemble& e, std::string& err) {
   hipEvent_t t0 = sc.ev[0], t1 = sc.ev[1], t2 = sc.ev[2];
   float best = 1e30f;
   int bty = 8, bzc = 8, bord = 1;
-  const int tys[4] = {kHallCode, 4, 8, 32}, zcs[3] = {4, 8, 16};
-  // IEQ_ENS_SHAPE=<code> restricts the search to one tile shape (1, 4, 8 or 32)
+  const int tys[5] = {kHallCode, kHallCode2, 4, 8, 32}, zcs[4] = {4, 8, 16, 32};
+  // IEQ_ENS_SHAPE=<code> restricts the search to one tile shape (1, 2, 4, 8 or 32)
   const char* force = std::getenv("IEQ_ENS_SHAPE");
   const int forced = force && *force ? std::atoi(force) : 0;
   for (int ty : tys)
     for (int zc : zcs)
       for (int ord = 0; ord < 2; ++ord) {
-        if ((ty == kHallCode && !e.hall1) || (forced && ty != forced)) continue;
+        if ((is_hall_code(ty) && !e.hall1) || (forced && ty != forced)) continue;
         int rc = ens_configure<R>(e, ty, zc, ord, err);
         if (rc) return rc;
         hipError_t r = ens_launch_steps<R>(e, e.stream);  // warm-up

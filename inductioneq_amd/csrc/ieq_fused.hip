@@ -503,6 +503,7 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
     a.two_h[d] = (R)(2.0 / h[d]);
   }
   a.rho = (R)mn[3];
+  for (int d = 0; d < 3; ++d) a.jsc[d] = (R)(1.0 / (h[d] * mn[3]));
   a.u[0] = (R)mn[0];
   a.u[1] = (R)mn[1];
   a.u[2] = (R)mn[2];

@@ -24,6 +24,9 @@
 namespace ieq {
 namespace {
 
+#ifndef IEQ_HALL_HSKIP
+#define IEQ_HALL_HSKIP 0
+#endif
 // Extra LDS row elements (bank spreading of the scattered halo accesses); 0 by default.
 #ifndef IEQ_HALL_PADB
 #define IEQ_HALL_PADB 0
@@ -97,26 +100,28 @@ template <int RR, int TX, int TY> __device__ inline void stage_item(int i, int& 
   }
 }
 
-// E = (curl_D B / rho) x B at LDS node (bx, by) of the B plane, with the z-derivatives of B_x and
-// B_y already formed (SBP_operator.cl:126-135, kernel_time_integrator.cl:147).
+// E = (curl_D B / rho) x B at LDS node (bx, by) of the B plane, with the undivided z-derivatives of
+// B_x and B_y already formed (SBP_operator.cl:126-135, kernel_time_integrator.cl:147).  Each
+// undivided derivative along d is scaled by a.jsc[d] = 1/(dx_d rho).
 template <typename R, int RR, int BY, int BX>
 __device__ __forceinline__ void emf_at(const FArgs<R>& a, R (*sB)[BY][BX], int bx, int by, R dzBx, R dzBy, R Bx, R By,
-                                       R Bz, R irho, R& Ex, R& Ey, R& Ez) {
+                                       R Bz, R& Ex, R& Ey, R& Ez) {
   constexpr int Q = 2 * RR + 1;
   R v[Q];
 #pragma unroll
   for (int k = 0; k < Q; ++k) v[k] = sB[1][by][bx - RR + k];
-  const R dxBy = deriv<R, RR>(a.c, v) * a.inv_h[0];
+  const R dxBy = deriv<R, RR>(a.c, v);
 #pragma unroll
   for (int k = 0; k < Q; ++k) v[k] = sB[2][by][bx - RR + k];
-  const R dxBz = deriv<R, RR>(a.c, v) * a.inv_h[0];
+  const R dxBz = deriv<R, RR>(a.c, v);
 #pragma unroll
   for (int k = 0; k < Q; ++k) v[k] = sB[0][by - RR + k][bx];
-  const R dyBx = deriv<R, RR>(a.c, v) * a.inv_h[1];
+  const R dyBx = deriv<R, RR>(a.c, v);
 #pragma unroll
   for (int k = 0; k < Q; ++k) v[k] = sB[2][by - RR + k][bx];
-  const R dyBz = deriv<R, RR>(a.c, v) * a.inv_h[1];
-  const R Jx = (dyBz - dzBy * a.inv_h[2]) * irho, Jy = (dzBx * a.inv_h[2] - dxBz) * irho, Jz = (dxBy - dyBx) * irho;
+  const R dyBz = deriv<R, RR>(a.c, v);
+  const R Jx = a.jsc[1] * dyBz - a.jsc[2] * dzBy, Jy = a.jsc[2] * dzBx - a.jsc[0] * dxBz,
+          Jz = a.jsc[0] * dxBy - a.jsc[1] * dyBx;
   Ex = Jy * Bz - Jz * By;
   Ey = Jz * Bx - Jx * Bz;
   Ez = Jx * By - Jy * Bx;
@@ -146,7 +151,6 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
   const Comps<R, 3> B = comps<R, 3>(a.Y, S);
   const R* Pp = MODE == 1 ? a.b2 : (a.P ? a.P : a.Y);
   const R* Qp = a.Q ? a.Q : a.Y;
-  const R irho = R(1) / a.rho;
 
   // E-halo column of this thread (threads past the last one repeat it and do not store)
   const bool has_h = tid < G::NH;
@@ -256,9 +260,10 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
     R r[3] = {R(0), R(0), R(0)};
     {
       R dB[3][3];   // undivided derivative sums: component c along direction d
-      R hsum[3];    // dissipation sums per component
+      R hsum[3];    // dissipation sums per component: w_0 B over the three directions, then w_|k| B(n + k e_d)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
+        if constexpr (HOD) hsum[c] = (a.c.w[0] * R(3)) * qb[c][RR + SK];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           const bool for_j = c != d;  // curl B uses every derivative across components
@@ -270,14 +275,21 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
                            : d == 0 ? sB[c][cy][cx - RR + k] : d == 1 ? sB[c][cy - RR + k][cx] : qb[c][SK + k];
           if (for_j || CONV) dB[c][d] = deriv<R, RR>(a.c, v);
           if constexpr (HOD) {
-            if (rhs_pe) hsum[c] = d == 0 ? hodsum<R, RR>(a.c, v) : hsum[c] + hodsum<R, RR>(a.c, v);
+            // the dissipation sums of the three directions in one chain (hodsum per direction, merged)
+            if (rhs_pe) {
+#pragma unroll
+              for (int k = 1; k <= RR; ++k) {
+                hsum[c] = hsum[c] + a.c.w[k] * v[RR - k];
+                hsum[c] = hsum[c] + a.c.w[k] * v[RR + k];
+              }
+            }
           }
         }
       }
       // J = curl_D B / rho, E = J x B  (SBP_operator.cl:126-135, kernel_time_integrator.cl:147)
-      const R Jx = (dB[2][1] * a.inv_h[1] - dB[1][2] * a.inv_h[2]) * irho;
-      const R Jy = (dB[0][2] * a.inv_h[2] - dB[2][0] * a.inv_h[0]) * irho;
-      const R Jz = (dB[1][0] * a.inv_h[0] - dB[0][1] * a.inv_h[1]) * irho;
+      const R Jx = a.jsc[1] * dB[2][1] - a.jsc[2] * dB[1][2];
+      const R Jy = a.jsc[2] * dB[0][2] - a.jsc[0] * dB[2][0];
+      const R Jz = a.jsc[0] * dB[1][0] - a.jsc[1] * dB[0][1];
       const R Bx = qb[0][RR + SK], By = qb[1][RR + SK], Bz = qb[2][RR + SK];
       const R Ex = Jy * Bz - Jz * By, Ey = Jz * Bx - Jx * Bz, Ez = Jx * By - Jy * Bx;
       sE[0][ty + RR][tx + RR] = Ex;
@@ -307,11 +319,14 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
         }
       }
     }
-    // E at the E-halo column
+    // E at the E-halo column (a wave whose threads all lie past the last halo column skips it)
+#if IEQ_HALL_HSKIP
+    if (has_h)
+#endif
     {
       R hx, hy, hz;
       emf_at<R, RR, G::BY, G::BX>(a, sB, hlx + 2 * RR, hly + 2 * RR, deriv<R, RR>(a.c, qh[0]), deriv<R, RR>(a.c, qh[1]),
-                                  qh[0][RR], qh[1][RR], shz, irho, hx, hy, hz);
+                                  qh[0][RR], qh[1][RR], shz, hx, hy, hz);
       if (has_h) {
         sE[0][hly + RR][hlx + RR] = hx;
         sE[1][hly + RR][hlx + RR] = hy;

@@ -24,6 +24,7 @@ template <typename R> struct FArgs {
   R inv_h[3];  // (R)(1/dx_d): Hall_volume and curl scaling
   R two_h[3];  // (R)(2/dx_d): convec_volume scaling
   R rho;       // uniform density (J = curl B / rho)
+  R jsc[3];    // (R)(1/(dx_d rho)): scaling of the undivided derivatives that form J
   R u[3];      // uniform velocity
   int conv;    // velocity non-zero: evaluate the convection term
   int src_zero;

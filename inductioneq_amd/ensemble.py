@@ -94,8 +94,10 @@ class Ensemble:
         ty, zc, order = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _capi.check(self.lib.ieq_ensemble_config(self.handle, ctypes.byref(ty), ctypes.byref(zc), ctypes.byref(order)),
                     self.program.handle)
-        if ty.value == 1:  # the one-kernel Hall stage (E formed in the stage kernel)
-            tile, name = ((32, 8) if self.program.info.real_bytes == 8 else (64, 8)), "hall"
+        if ty.value in (1, 2):  # the one-kernel Hall stage (E formed in the stage kernel), two tiles
+            f64 = self.program.info.real_bytes == 8
+            tile = ((32, 8) if f64 else (64, 8)) if ty.value == 1 else ((16, 16) if f64 else (32, 8))
+            name = "hall_%dx%d" % tile
         else:
             tile = {4: (64, 4), 8: (64, 8), 32: (32, 8)}.get(ty.value, (64, ty.value))  # tile shape code -> (x, y)
             name = "%dx%d" % tile

@@ -72,13 +72,13 @@ def _ensemble(ops, hall, diss, integrator, real, seeds):
     return cfgs, ens, Bs
 
 
-@pytest.mark.parametrize("shape", ["auto", "1", "32"])
+@pytest.mark.parametrize("shape", ["auto", "1", "2", "32"])
 @pytest.mark.parametrize("integrator", ["SSPRK33", "CarpenterKennedy2N54", "ToulorgeDesmet2N84F"])
 @pytest.mark.parametrize("ops,hall,diss", [("6thOrder", True, "HIGH_ORDER"), ("4thOrder", False, "HIGH_ORDER"),
                                            ("2ndOrder", True, None)])
 def test_ensemble_members_match_oracle(integrator, ops, hall, diss, shape, monkeypatch):
-    """Tile shape chosen by the tuner, or forced (IEQ_ENS_SHAPE): 1 the one-kernel Hall stage (E formed in the
-    stage kernel), 32 the EMF kernel + stage kernel pair."""
+    """Tile shape chosen by the tuner, or forced (IEQ_ENS_SHAPE): 1 and 2 the one-kernel Hall stage (E formed in
+    the stage kernel; FP64 tiles 32 x 8 and 16 x 16), 32 the EMF kernel + stage kernel pair."""
     import torch
 
     if shape != "auto":
@@ -89,9 +89,9 @@ def test_ensemble_members_match_oracle(integrator, ops, hall, diss, shape, monke
     ens.step(nsteps)
     torch.cuda.synchronize()
     tile = ens.config()["tile"]
-    assert tile in ("64x4", "64x8", "32x8", "hall")
-    if shape == "1":
-        assert tile == ("hall" if hall else "64x8")
+    assert tile in ("64x4", "64x8", "32x8", "hall_32x8", "hall_16x16")
+    if shape in ("1", "2"):
+        assert tile == (("hall_32x8" if shape == "1" else "hall_16x16") if hall else "64x8")
     elif shape == "32":
         assert tile == "32x8"
     for i, (cfg, B) in enumerate(zip(cfgs, Bs)):
