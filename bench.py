@@ -268,6 +268,10 @@ def run_secondary(a):
     h = run_hall(256, "double", "CarpenterKennedy2N54", 10, 2)
     out["hall256_fp64"] = {k: h[k] for k in ("value", "us_per_stage", "achieved", "frac")}
     out["hall256_fp64"]["unit"] = "Mcell-updates/s"
+    t256 = stored_traffic("hall256_fp64_traffic.json", "double")
+    if t256:  # HBM-side bytes per stage (the 768 MiB working set does not fit the 256 MB cache)
+        out["hall256_fp64"]["traffic"] = round(t256["bytes_per_stage"])
+        out["hall256_fp64"]["traffic_source"] = t256["source"]
     del h
     torch.cuda.empty_cache()
     h = run_hall(128, "float", "CarpenterKennedy2N54", 40, 5)
@@ -446,10 +450,15 @@ def run_adjoint(a, steps, n):
             uf[k, :N3] = u[k]
         args = [b1, prog.vector(), prog.vector(), prog.vector(), uf, prog.scalar(1.0 / ch), prog.time_buffer()]
         prog.step("SSPRK33", 2, args)  # warm-up
-        b1.copy_(b0)
-        t_plain = prog.step("SSPRK33", steps, args, timed=True)
+        t_plain = 1e30
+        for _ in range(3):  # best of three plain solves
+            b1.copy_(b0)
+            t_plain = min(t_plain, prog.step("SSPRK33", steps, args, timed=True))
         bT = solver.forward(b0, u, ch)
         t_fwd = solver.last_forward_seconds
+        # the cost ratio is taken against the faster of the two forward solves the library offers (ieq_step,
+        # and the adjoint solver's own forward sweep, which also stores the snapshots)
+        t_plain = min(t_plain, t_fwd)
         lam = (bT - tgt) * (h ** 3)
         g, gc, gu = solver.backward(lam)
         t_bwd = solver.last_backward_seconds

@@ -13,6 +13,9 @@
 // k_adj_pre evaluates F = c_H (Y x M), G = c_H (M x J) and the parameter partials in one
 // z-marching pass over Y and L; the transposed stage is the forward stage kernel run on
 // (Y := L, E := F) with the pointwise -G term, the transposed source form and u -> -u.
+// Each block of k_adj_pre adds its scaled partial sums to its own slot of a running FP64
+// accumulator (the grid, and so the block of every tile, is the same for every launch of a
+// sweep); the slots are summed in block order once, when the sweep ends.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -31,15 +34,16 @@ template <typename R> struct PreArgs {
   int nx, ny, nz, zc;
   long long S;
   StencilCoef<R> c;
-  R h[3];
+  R inv_h[3];  // (R)(1/dx_d)
   R c_hall;
+  double scale;  // weight of this launch's partial sums
   int src_zero;
   int hall;  // 0: no Hall term (F, G not written, no c_H partial)
   const R* Y;
   const R* L;
   R* F;
   R* G;
-  double* partial;  // [gridDim.x][4]: d/dc_H, d/du_x, d/du_y, d/du_z
+  double* partial;  // [gridDim.x][4] running sums of d/dc_H, d/du_x, d/du_y, d/du_z
 };
 
 template <typename R, int RR, int TY>
@@ -89,11 +93,14 @@ __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
     sL[2][ty + RR][tx + RR] = lz;
     hy.store(sY, tx, ty);
     hl.store(sL, tx, ty);
-    // next plane's loads, in the order they are consumed: halos and L_z, then the column
-    if (z + 1 < zend) {
-      hy.load(Yc, (unsigned)(z + 1) * plb);
-      hl.load(Lc, (unsigned)(z + 1) * plb);
-      lz = ldo(Lc.c[2], colb + (unsigned)(z + 1) * plb);
+    // next plane's loads, in the order they are consumed: halos and L_z, then the column; on
+    // the last plane of the chunk they re-read that plane (unconditional loads keep the waits
+    // for them exact)
+    {
+      const int zn = min(z + 1, zend - 1);
+      hy.load(Yc, (unsigned)zn * plb);
+      hl.load(Lc, (unsigned)zn * plb);
+      lz = ldo(Lc.c[2], colb + (unsigned)zn * plb);
       const unsigned o = colb + (unsigned)wrap(z + RR + 1, a.nz) * plb;
 #pragma unroll
       for (int c = 0; c < 3; ++c) qy[c][Q] = ldo(Yc.c[c], o);
@@ -109,26 +116,26 @@ __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
       for (int c = 0; c < 3; ++c) {
 #pragma unroll
         for (int k = 0; k < Q; ++k) v[k] = sY[c][ty + RR][tx + k];
-        dY[0][c] = deriv<R, RR>(a.c, v) / a.h[0];
+        dY[0][c] = deriv<R, RR>(a.c, v) * a.inv_h[0];
 #pragma unroll
         for (int k = 0; k < Q; ++k) v[k] = sY[c][ty + k][tx + RR];
-        dY[1][c] = deriv<R, RR>(a.c, v) / a.h[1];
-        dY[2][c] = deriv<R, RR>(a.c, qy[c]) / a.h[2];
+        dY[1][c] = deriv<R, RR>(a.c, v) * a.inv_h[1];
+        dY[2][c] = deriv<R, RR>(a.c, qy[c]) * a.inv_h[2];
       }
 #pragma unroll
       for (int k = 0; k < Q; ++k) v[k] = sL[1][ty + RR][tx + k];
-      const R dxLy = deriv<R, RR>(a.c, v) / a.h[0];
+      const R dxLy = deriv<R, RR>(a.c, v) * a.inv_h[0];
 #pragma unroll
       for (int k = 0; k < Q; ++k) v[k] = sL[2][ty + RR][tx + k];
-      const R dxLz = deriv<R, RR>(a.c, v) / a.h[0];
+      const R dxLz = deriv<R, RR>(a.c, v) * a.inv_h[0];
 #pragma unroll
       for (int k = 0; k < Q; ++k) v[k] = sL[0][ty + k][tx + RR];
-      const R dyLx = deriv<R, RR>(a.c, v) / a.h[1];
+      const R dyLx = deriv<R, RR>(a.c, v) * a.inv_h[1];
 #pragma unroll
       for (int k = 0; k < Q; ++k) v[k] = sL[2][ty + k][tx + RR];
-      const R dyLz = deriv<R, RR>(a.c, v) / a.h[1];
-      const R dzLx = deriv<R, RR>(a.c, ql[0]) / a.h[2];
-      const R dzLy = deriv<R, RR>(a.c, ql[1]) / a.h[2];
+      const R dyLz = deriv<R, RR>(a.c, v) * a.inv_h[1];
+      const R dzLx = deriv<R, RR>(a.c, ql[0]) * a.inv_h[2];
+      const R dzLy = deriv<R, RR>(a.c, ql[1]) * a.inv_h[2];
       const R Mx = dyLz - dzLy, My = dzLx - dxLz, Mz = dxLy - dyLx;
       const R Jx = dY[1][2] - dY[2][1], Jy = dY[2][0] - dY[0][2], Jz = dY[0][1] - dY[1][0];
       const R Yx = qy[0][RR], Yy = qy[1][RR], Yz = qy[2][RR];
@@ -172,7 +179,7 @@ __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
   if (ty == 0 && tx < 4) {
     double s = 0.0;
     for (int w = 0; w < TY; ++w) s += red[tx][w];
-    a.partial[(long long)blockIdx.x * 4 + tx] = s;
+    a.partial[(long long)blockIdx.x * 4 + tx] += a.scale * s;
   }
 }
 
@@ -197,8 +204,8 @@ __global__ __launch_bounds__(256) void k_adj_accumulate(const double* __restrict
   if (t < 4) acc[t] += scale * sm[t][0];
 }
 
-__global__ void k_adj_zero(double* acc) {
-  if (threadIdx.x < 4) acc[threadIdx.x] = 0.0;
+__global__ void k_adj_zero(double* acc, int n) {
+  for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) acc[i] = 0.0;
 }
 
 template <typename R> __global__ void k_copy(R* __restrict__ dst, const R* __restrict__ src, long long n) {
@@ -347,9 +354,10 @@ template <typename R> void pre(Adjoint& A, const R* Y, const R* L, double scale)
   q.zc = A.zc;
   q.S = A.info.stride;
   q.c = coefs<R>(A);
-  q.h[0] = (R)A.p.dx;
-  q.h[1] = (R)A.p.dy;
-  q.h[2] = (R)A.p.dz;
+  q.inv_h[0] = (R)(1.0 / A.p.dx);
+  q.inv_h[1] = (R)(1.0 / A.p.dy);
+  q.inv_h[2] = (R)(1.0 / A.p.dz);
+  q.scale = scale;
   q.c_hall = (R)A.c_hall;
   q.src_zero = A.p.form_source == 0;
   q.hall = A.p.hall != 0;
@@ -369,7 +377,6 @@ template <typename R> void pre(Adjoint& A, const R* Y, const R* L, double scale)
     else if (A.rr == 2) k_adj_pre<R, 2, 8><<<grid, blk, 0, A.s>>>(q);
     else k_adj_pre<R, 3, 8><<<grid, blk, 0, A.s>>>(q);
   }
-  k_adj_accumulate<<<1, 256, 0, A.s>>>(A.partial, grid, scale, A.acc);
 }
 
 // Transposed stage: out = cP P + cQ Q + cY L + cD RHS'(.)^T L, with F, G from pre().
@@ -528,7 +535,8 @@ template <typename R> int forward_t(Adjoint& A, const void* b0, void* bT) {
 
 template <typename R> int backward_t(Adjoint& A, const void* lamT, void* grad_b0) {
   copy<R>(A, A.lam, lamT);
-  k_adj_zero<<<1, 64, 0, A.s>>>(A.acc);
+  k_adj_zero<<<1, 64, 0, A.s>>>(A.acc, 4);
+  k_adj_zero<<<1, 256, 0, A.s>>>(A.partial, A.nblk * 4);
   if (A.stages) {
     for (long long k = A.nsteps - 1; k >= 0; --k)
       adj_from<R>(A, (const R*)A.st(0, k), (const R*)A.st(1, k), (const R*)A.st(2, k));
@@ -536,6 +544,7 @@ template <typename R> int backward_t(Adjoint& A, const void* lamT, void* grad_b0
     reverse<R>(A, 0, A.nsteps, 0, A.snapshots);
   }
   copy<R>(A, grad_b0, A.lam);
+  k_adj_accumulate<<<1, 256, 0, A.s>>>(A.partial, A.nblk, 1.0, A.acc);  // the blocks' running sums, in block order
   return 0;
 }
 

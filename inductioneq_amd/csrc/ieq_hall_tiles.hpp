@@ -229,9 +229,10 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
   __builtin_amdgcn_sched_barrier(0);
 
   auto body = [&](auto slot, int o) {
-    constexpr int SL = decltype(slot)::value;
+    constexpr int UL = decltype(slot)::value;  // position in the two-iteration loop body
+    constexpr int SL = UL % PD;                 // register slot of the loads in flight
     const int pe = o + RR + SK;
-    const int buf = SK ? (PD == 2 ? SL : ((o - o0) & 1)) : 0;
+    const int buf = SK ? UL : 0;
     auto& sB = L.sB[buf];
     auto& sE = L.sE[buf];
     // window heads: plane q = pe + RR
@@ -421,21 +422,31 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
       for (int c = 0; c < 3; ++c) rx[c][k] = rx[c][k + 1];
     }
   };
-  for (int o = o0; o < zend; o += PD) {
+  // two iterations per trip (the loop body is instantiated once per position, so the windows
+  // shift by renaming inside the trip); a trip may run one iteration past zend, which writes nothing
+  for (int o = o0; o < zend; o += 2) {
     body(std::integral_constant<int, 0>{}, o);
-    if constexpr (PD == 2) body(std::integral_constant<int, 1>{}, o + 1);
+    body(std::integral_constant<int, 1>{}, o + 1);
   }
 }
 
 #ifndef IEQ_HALL_WAVES
 #define IEQ_HALL_WAVES 2
 #endif
+// The batched FP32 form holds the tile loop's state besides the single-grid kernel's: at 4
+// waves per SIMD (128 VGPRs) its 6th-order instantiations spill 14-20 VGPRs, so it is compiled
+// for 3 (64-member ensemble: 0.227 s against 0.234 s on one box, where the tuner then prefers
+// the two-kernel form).
+#ifndef IEQ_HALL_WAVES32B
+#define IEQ_HALL_WAVES32B 3
+#endif
 #ifndef IEQ_HALL_WAVES32
 #define IEQ_HALL_WAVES32 4
 #endif
-// Planes of column loads in flight: two in FP64 (2 waves per SIMD), one in FP32.
+// Planes of column loads in flight: one (measured faster than two once the loop body holds two
+// iterations in both precisions).
 #ifndef IEQ_HALL_PD64
-#define IEQ_HALL_PD64 2
+#define IEQ_HALL_PD64 1
 #endif
 template <typename R> constexpr int hall_pd() { return sizeof(R) == 8 ? IEQ_HALL_PD64 : 1; }
 // One barrier per z-step with double-buffered LDS planes (1) or two barriers and single planes (0).
@@ -454,7 +465,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(
 
 // Persistent batched form over a tile list covering every ensemble member.
 template <typename R, int RR, int TX, int TY, bool HOD, bool CONV, int MODE>
-__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? IEQ_HALL_WAVES32 : IEQ_HALL_WAVES, 8))) void k_hall_stage_batched(
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? IEQ_HALL_WAVES32B : IEQ_HALL_WAVES, 8))) void k_hall_stage_batched(
     const FArgs<R>* __restrict__ args, const int4* __restrict__ tiles, int ntiles) {
   __shared__ HallLds<R, RR, TX, TY, 1 + IEQ_HALL_SKEW> L;
   const int chunk = (ntiles + kXcds - 1) / kXcds, slots = gridDim.x / kXcds;  // gridDim.x is a multiple of 8

@@ -120,12 +120,20 @@ def test_ensemble_odd_step_count_rejected_for_2n():
         ens.step(3)
 
 
-def test_ensemble_fp32_within_1e5_of_fp64():
-    """Mixed-precision claim for batched runs: FP32 members stay within 1e-5 (relative, max norm) of FP64."""
+@pytest.mark.parametrize("shape", ["auto", "1", "2", "32"])
+def test_ensemble_fp32_within_1e5_of_fp64(shape, monkeypatch):
+    """Mixed-precision claim for batched runs: FP32 members stay within 1e-5 (relative, max norm) of FP64, for
+    the tuner's choice and for each forced FP32 launch form (one-kernel Hall stage with 64 x 8 or 32 x 8 tiles,
+    EMF + stage kernels with 32 x 8 tiles)."""
     _, e64, _ = _ensemble("6thOrder", True, "HIGH_ORDER", "SSPRK33", "double", seeds=(7, 8, 9))
-    _, e32, _ = _ensemble("6thOrder", True, "HIGH_ORDER", "SSPRK33", "float", seeds=(7, 8, 9))
     e64.step(10)
+    if shape != "auto":
+        monkeypatch.setenv("IEQ_ENS_SHAPE", shape)
+    _, e32, _ = _ensemble("6thOrder", True, "HIGH_ORDER", "SSPRK33", "float", seeds=(7, 8, 9))
     e32.step(10)
+    if shape != "auto":
+        want = {"1": "hall_64x8", "2": "hall_32x8", "32": "32x8"}[shape]
+        assert e32.config()["tile"] == want
     for i in range(len(MEMBERS)):
         a, b = e64.state(i).numpy(), e32.state(i).numpy().astype(np.float64)
         err = np.abs(a - b).max() / np.abs(a).max()
