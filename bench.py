@@ -282,6 +282,7 @@ def run_secondary(a):
     e = run_ensemble(a, 200)
     out["ensemble64"] = {"value": e["value"], "unit": e["unit"], "speedup_vs_per_instance_fp64": e["speedup_vs_per_instance_fp64"],
                          "max_rel_err_vs_fp64": e["max_rel_err_vs_fp64"], "roofline_frac": e["roofline"]["frac"],
+                         "traffic": e["roofline"]["traffic"],
                          "per_instance_fp64": e["per_instance_fp64"], "launch": e["config"]["launch"]}
     torch.cuda.empty_cache()
     d = run_adjoint(a, 200, 96)
@@ -374,6 +375,15 @@ def run_ensemble(a, steps):
     value = ncell * stages / sec32 / 1e6
     rb = 4
     alg = 64 // 2 * ncell  # SSPRK33 average 8/3 vector fields per stage: 32 B/cell FP32
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "ensemble64_fp32_traffic.json")
+    if a.members == 64 and os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if t["launch"]["tile"] == cfg_e["tile"] and t["launch"]["z_chunk"] == cfg_e["z_chunk"]:
+            # HBM-side bytes per stage of the profiled launch form: SSPRK33 runs stage 1 in update mode 3 and
+            # stages 2-3 in mode 2
+            by = {int(k.rsplit(",", 1)[1].strip(" >")): v["bytes"] for k, v in t["kernels"].items()}
+            traffic = round((by[3] + 2 * by[2]) / 3)
     out = {
         "metric": "Mcell-updates/s per RK stage",
         "value": round(value, 3),
@@ -390,7 +400,8 @@ def run_ensemble(a, steps):
         "config": {"workload": "ensemble_%d_hall_periodic_6th_HOD_SSPRK33" % a.members, "members": a.members,
                    "cells": ncell, "sizes": [int(v) for v in sizes], "launch": cfg_e},
         "roofline": {"bound": "hbm", "achieved": round(alg / (sec32 / stages) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(alg / (sec32 / stages) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(alg / (sec32 / stages) / 1e9 / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/ensemble64_fp32_traffic.json" if traffic else None,
                      "kernel": "one batched RK stage (%s), 32 B/cell algorithmic"
                                % ("k_hall_stage_batched" if cfg_e["tile"].startswith("hall") else "k_emf_batched + k_stage_batched")},
         "per_instance_fp64": {"seconds": round(sec64, 4), "value": round(ncell * stages / sec64 / 1e6, 3)},

@@ -330,14 +330,19 @@ template <typename R> int ens_tune(Ensemble& e, std::string& err) {
   float best = 1e30f;
   int bty = 8, bzc = 8, bord = 1;
   const int tys[5] = {kHallCode, kHallCode2, 4, 8, 32}, zcs[4] = {4, 8, 16, 32};
-  // IEQ_ENS_SHAPE=<code> restricts the search to one tile shape (1, 2, 4, 8 or 32)
-  const char* force = std::getenv("IEQ_ENS_SHAPE");
-  const int forced = force && *force ? std::atoi(force) : 0;
+  // IEQ_ENS_SHAPE=<code> restricts the search to one tile shape (1, 2, 4, 8 or 32), IEQ_ENS_ZC
+  // to one z-chunk and IEQ_ENS_ORDER to one tile order (0 member order, 1 largest first)
+  auto env_or = [](const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    return v && *v ? std::atoi(v) : dflt;
+  };
+  const int forced = env_or("IEQ_ENS_SHAPE", 0), forced_zc = env_or("IEQ_ENS_ZC", 0), forced_ord = env_or("IEQ_ENS_ORDER", -1);
   bool warm = false;
   for (int ty : tys)
     for (int zc : zcs)
       for (int ord = 0; ord < 2; ++ord) {
         if ((is_hall_code(ty) && !e.hall1) || (forced && ty != forced)) continue;
+        if ((forced_zc > 0 && zc != forced_zc) || (forced_ord >= 0 && ord != forced_ord)) continue;
         int rc = ens_configure<R>(e, ty, zc, ord, err);
         if (rc) return rc;
         hipError_t r = ens_launch_steps<R>(e, e.stream);  // warm-up
