@@ -118,7 +118,7 @@ def test_bounded_stage_all_integrators(integrator):
     _check(cfg, prog, db, hb, integrator, integrator)
 
 
-@pytest.mark.parametrize("zc", ["2", "5"])
+@pytest.mark.parametrize("zc", ["1", "2", "5"])
 @pytest.mark.parametrize("ops", ["4thOrder", "6thOrderExtended"])
 def test_bounded_stage_forced_z_chunks(zc, ops):
     """Many z-chunks: chunk boundaries inside and next to the boundary planes of z."""
