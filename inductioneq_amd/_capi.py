@@ -118,6 +118,13 @@ def load() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError("libinductioneq_hip.so is not built (%s); run `python -m inductioneq_amd.build` "
                           "or __graft_entry__.build()" % LIB_PATH)
+    try:
+        # torch ships its own HIP runtime; with torch loaded first the library binds to that same runtime, so
+        # device memory from torch tensors and the library's own calls share one runtime (loaded the other way
+        # round, the library's first allocation found no device once torch had started)
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     lib.ieq_program_build.argtypes = [ctypes.POINTER(P), ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int]

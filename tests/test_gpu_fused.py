@@ -197,7 +197,7 @@ def test_rk_step_list_takes_fused_path(integrator):
         assert prog.last_step_fused == (no_fused == "0")
         runs.append(args)
     fz, ref = runs
-    tol = 1e-12 * (20 if integrator == "SSPRK104" else 1)
+    tol = 1e-12
     for i in range(nf):
         e = _relerr(fz[i][:, : cfg.num_nodes].cpu().numpy(), ref[i][:, : cfg.num_nodes].cpu().numpy())
         # work registers of the low-storage schemes hold dt-scaled increments (cancellation)
