@@ -195,6 +195,22 @@ int hip_fail(ieq_program* prog, hipError_t e, const char* what) {
     if (_e != hipSuccess) return hip_fail((prog), _e, #call); \
   } while (0)
 
+// Scratch device memory and timing events owned by one call: released on every return path
+// (hipFree waits for the work still using the buffer).
+struct ScopedDev {
+  void* p = nullptr;
+  ~ScopedDev() {
+    if (p) (void)hipFree(p);
+  }
+};
+struct ScopedEvents {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  ~ScopedEvents() {
+    for (hipEvent_t v : e)
+      if (v) (void)hipEventDestroy(v);
+  }
+};
+
 int parse_build(ieq_program* pr, const char* settings, const char* const* modules, int nmod) {
   std::map<std::string, std::string> defs;
   if (settings) {
@@ -623,13 +639,13 @@ int ieq_upload_real4(ieq_program* prog, void* dst, const void* host_aos, size_t 
   if (!prog || !dst || !host_aos) return IEQ_ERR_ARGS;
   if ((long long)n > prog->info.stride) return prog->fail(IEQ_ERR_ARGS, "upload larger than the field");
   const size_t rb = prog->info.real_bytes;
-  void* tmp = nullptr;
-  HIP_TRY(prog, hipMalloc(&tmp, 4 * n * rb + 1));
+  ScopedDev buf;
+  HIP_TRY(prog, hipMalloc(&buf.p, 4 * n * rb + 1));
+  void* tmp = buf.p;
   HIP_TRY(prog, hipMemcpyAsync(tmp, host_aos, 4 * n * rb, hipMemcpyHostToDevice, prog->stream));
   if (rb == 8) ieq::gen_aos_to_soa<double>(prog->stream, (const double*)tmp, (double*)dst, (long long)n, prog->info.stride);
   else ieq::gen_aos_to_soa<float>(prog->stream, (const float*)tmp, (float*)dst, (long long)n, prog->info.stride);
   HIP_TRY(prog, hipStreamSynchronize(prog->stream));
-  HIP_TRY(prog, hipFree(tmp));
   return IEQ_OK;
 }
 
@@ -637,13 +653,13 @@ int ieq_download_real4(ieq_program* prog, void* host_aos, const void* src, size_
   if (!prog || !src || !host_aos) return IEQ_ERR_ARGS;
   if ((long long)n > prog->info.stride) return prog->fail(IEQ_ERR_ARGS, "download larger than the field");
   const size_t rb = prog->info.real_bytes;
-  void* tmp = nullptr;
-  HIP_TRY(prog, hipMalloc(&tmp, 4 * n * rb + 1));
+  ScopedDev buf;
+  HIP_TRY(prog, hipMalloc(&buf.p, 4 * n * rb + 1));
+  void* tmp = buf.p;
   if (rb == 8) ieq::gen_soa_to_aos<double>(prog->stream, (const double*)src, (double*)tmp, (long long)n, prog->info.stride);
   else ieq::gen_soa_to_aos<float>(prog->stream, (const float*)src, (float*)tmp, (long long)n, prog->info.stride);
   HIP_TRY(prog, hipMemcpyAsync(host_aos, tmp, 4 * n * rb, hipMemcpyDeviceToHost, prog->stream));
   HIP_TRY(prog, hipStreamSynchronize(prog->stream));
-  HIP_TRY(prog, hipFree(tmp));
   return IEQ_OK;
 }
 
@@ -817,22 +833,21 @@ int clean_div_t(ieq_program* pr, R* b, R* phi, double thr, double rel, int max_i
   if (!tab) return pr->fail(IEQ_ERR_SETTINGS, "no -Laplacian table for this operator set");
   std::memset(res, 0, sizeof(*res));
   const size_t sbytes = (size_t)I.stride * sizeof(R);
-  R* tmp = nullptr;  // scalar work field + reduction partials
-  HIP_TRY(pr, hipMalloc(&tmp, 4 * sbytes + (size_t)I.num_groups * sizeof(R) + 64));
+  ScopedDev buf;  // scalar work fields + reduction partials
+  HIP_TRY(pr, hipMalloc(&buf.p, 4 * sbytes + (size_t)I.num_groups * sizeof(R) + 64));
+  R* tmp = (R*)buf.p;
   R *r = tmp, *u = tmp + I.stride, *q = tmp + 2 * I.stride, *dv = tmp + 3 * I.stride, *part = tmp + 4 * I.stride;
   int rc = ieq::cg_run_t<R>(p, I, tab, nbl, wl, pr->stream, &pr->cg, b, phi, thr, rel, max_it, 16, res, pr->err);
-  if (rc < 0) {
-    (void)hipFree(tmp);
-    return IEQ_ERR_HIP;
-  }
+  if (rc < 0) return IEQ_ERR_HIP;
   const uint32_t g_dc[3] = {(uint32_t)(p.nx - 2 * p.bnodes), (uint32_t)(p.ny - 2 * p.bnodes), (uint32_t)(p.nz - 2 * p.bnodes)};
   if (rc == 1) {  // reference kernel sequence with host-side scalars (clean_div_CG.m:31-76)
     HIP_TRY(pr, hipMemsetAsync(tmp, 0, 3 * sbytes, pr->stream));
     HIP_TRY(pr, hipMemsetAsync(phi, 0, sbytes, pr->stream));
     R* coef = part + I.num_groups;  // 1-element coefficient buffer
-    hipEvent_t e0, e1;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
+    ScopedEvents ev;
+    HIP_TRY(pr, hipEventCreate(&ev.e[0]));
+    HIP_TRY(pr, hipEventCreate(&ev.e[1]));
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     (void)hipEventRecord(e0, pr->stream);
     ieq::gen_div<R>(p, I.opset, pr->stream, g_dc, b, r, p.bnodes);
     double rr = 0, old = 1.0, uq = 0;
@@ -862,8 +877,6 @@ int clean_div_t(ieq_program* pr, R* b, R* phi, double thr, double rel, int max_i
     (void)hipEventSynchronize(e1);
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     res->iterations = it;
     res->residual_norm = std::sqrt(rr);
     res->seconds = ms * 1e-3;
@@ -877,7 +890,6 @@ int clean_div_t(ieq_program* pr, R* b, R* phi, double thr, double rel, int max_i
   if ((rc = reduce_host<R>(pr, 2, dv, nullptr, part, &dn))) return rc;
   res->divergence_norm_final = std::sqrt(std::max(dn, 0.0));
   HIP_TRY(pr, hipStreamSynchronize(pr->stream));
-  HIP_TRY(pr, hipFree(tmp));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(pr, e, "clean_div_cg");
   return IEQ_OK;
