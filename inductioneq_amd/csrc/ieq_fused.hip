@@ -241,6 +241,16 @@ int env_int(const char* k, int dflt) {
   return v && *v ? std::atoi(v) : dflt;
 }
 
+}  // namespace
+
+int hall_shape_for(long long nx, long long ny, long long nz, int real_bytes) {
+  const int forced = env_int("IEQ_HALL_TILE", -1);
+  if (forced == 0 || forced == 1) return forced;
+  return real_bytes == 8 && 3LL * 3 * nx * ny * nz * real_bytes <= 200LL * 1000 * 1000 ? 1 : 0;
+}
+
+namespace {
+
 // Central-row coefficients of the program's operator set (host copies of the tables).
 #undef IEQ_TABLE_STORAGE
 struct HostOp {
@@ -544,6 +554,7 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
     zc_emf = env_int("IEQ_EMF_ZC", 0);
     if (zc_emf < 1) zc_emf = pick_zc_emf<R>(rr, ty, a);
   }
+  const int hshape = hall_shape_for(p.nx, p.ny, p.nz, (int)sizeof(R));
   if (one_kernel) {
     zc_hall = env_int("IEQ_FUSED_ZC", 0);
     if (zc_hall < 1) {
@@ -552,7 +563,7 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
       t.mode = plan.back().kind == 4 ? 1 : 0;
       t.Y = b1;
       t.b2 = F[1];
-      zc_hall = pick_zc_hall<R>(rr, t, hod);
+      zc_hall = pick_zc_hall<R>(rr, t, hod, hshape);
     }
   }
   const R* y_last = nullptr;  // field differentiated by the last stage (curl B / rho of it on request)
@@ -582,7 +593,7 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
         if (one_kernel) {
           FArgs<R> ah = a;
           ah.zc = zc_hall;
-          launch_hall_stage<R>(rr, ah, hod, s);
+          launch_hall_stage<R>(rr, ah, hod, s, hshape);
         } else {
           stage<R>(rr, ty, a, p.hall, hod, s);
         }
@@ -606,7 +617,7 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
         if (one_kernel) {
           FArgs<R> ah = a;
           ah.zc = zc_hall;
-          launch_hall_stage<R>(rr, ah, hod, s);
+          launch_hall_stage<R>(rr, ah, hod, s, hshape);
         } else {
           stage<R>(rr, ty, a, p.hall, hod, s);
         }

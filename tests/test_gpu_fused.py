@@ -5,7 +5,7 @@
   through ieq_step, which picks the production tile shape and occupancy-sized z-chunks;
   FP64 against the FP64 oracle to 1e-12, FP32 against the FP32 oracle to 2e-5.
 * forced z-chunks (IEQ_FUSED_ZC / IEQ_EMF_ZC = 2, 3, 4) so every tile runs the multi-chunk
-  lead-in and a partial last chunk.
+  lead-in and a partial last chunk, for both tiles of the one-kernel stage (IEQ_HALL_TILE).
 * the exact solution B0 + eps*b(x - (u + m B0/rho) t) of the implemented equation for an ABC
   field b (curl b = m b) with uniform u and rho: the fused path converges to it at the
   operator's order.
@@ -89,9 +89,13 @@ def _oracle_steps(cfg, B, integrator, nsteps, u, rho):
     return hb, ht
 
 
-@pytest.mark.parametrize("real", ["double", "float"])
-def test_hall128_step_matches_oracle(real):
-    """cfg1 at 128^3: one CK2N54 step through ieq_step (production launch shape) vs the oracle."""
+_HALL128_ORACLE = {}
+
+
+@pytest.mark.parametrize("real,tile", [("double", None), ("double", 0), ("float", None)])
+def test_hall128_step_matches_oracle(real, tile):
+    """cfg1 at 128^3: one CK2N54 step through ieq_step (production launch shape; in FP64 also with the tile
+    that larger grids take, IEQ_HALL_TILE=0) vs the oracle."""
     import torch
 
     from inductioneq_amd.program import Program
@@ -102,10 +106,14 @@ def test_hall128_step_matches_oracle(real):
     cfg.dt = hall_dt(B0, (cfg.dx, cfg.dy, cfg.dz), 6, c_hall=1.0, cfl=0.5)
     prog = Program(cfg)
     args = _device_args(prog, B0, 2, (0.0, 0.0, 0.0), 1.0)
-    prog.step("CarpenterKennedy2N54", 1, args)
-    torch.cuda.synchronize()
+    env = {} if tile is None else {"IEQ_HALL_TILE": tile}
+    with _Env(**env):
+        prog.step("CarpenterKennedy2N54", 1, args)
+        torch.cuda.synchronize()
     assert prog.last_step_fused
-    hb, ht = _oracle_steps(cfg, B0, "CarpenterKennedy2N54", 1, (0.0, 0.0, 0.0), 1.0)
+    if real not in _HALL128_ORACLE:
+        _HALL128_ORACLE[real] = _oracle_steps(cfg, B0, "CarpenterKennedy2N54", 1, (0.0, 0.0, 0.0), 1.0)
+    hb, ht = _HALL128_ORACLE[real]
     got = args[0][:, : cfg.num_nodes].double().cpu().numpy().T
     want = hb[0][: cfg.num_nodes, :3].astype(np.float64)
     e = _relerr(got, want)
@@ -116,13 +124,14 @@ def test_hall128_step_matches_oracle(real):
     assert abs(float(args[-1][0]) - float(ht[0])) <= (1e-15 if real == "double" else 1e-7)
 
 
-@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("split,tile", [(0, 0), (0, 1), (1, 0)])
 @pytest.mark.parametrize("real", ["double", "float"])
 @pytest.mark.parametrize("zc", [2, 3, 4])
 @pytest.mark.parametrize("integrator", ["CarpenterKennedy2N54", "SSPRK33"])
-def test_forced_z_chunks_match_oracle(real, zc, integrator, split):
+def test_forced_z_chunks_match_oracle(real, zc, integrator, split, tile):
     """Every tile in a multi-chunk z-march (chunk lead-in planes, partial last chunk) vs the oracle, for the
-    one-kernel Hall stage and the two-kernel form (IEQ_FUSED_SPLIT=1: EMF kernel + stage kernel)."""
+    one-kernel Hall stage with either tile (IEQ_HALL_TILE: FP64 32 x 8 / 16 x 16, FP32 64 x 8 / 32 x 8) and the
+    two-kernel form (IEQ_FUSED_SPLIT=1: EMF kernel + stage kernel)."""
     import torch
 
     from inductioneq_amd.program import Program
@@ -134,7 +143,7 @@ def test_forced_z_chunks_match_oracle(real, zc, integrator, split):
     prog = Program(cfg)
     nf = 2 if integrator == "CarpenterKennedy2N54" else 3
     args = _device_args(prog, B0, nf, (0.3, -0.2, 0.1), 1.25)
-    with _Env(IEQ_FUSED_ZC=zc, IEQ_EMF_ZC=zc, IEQ_FUSED_SPLIT=split):
+    with _Env(IEQ_FUSED_ZC=zc, IEQ_EMF_ZC=zc, IEQ_FUSED_SPLIT=split, IEQ_HALL_TILE=tile):
         prog.step(integrator, 2, args)
         torch.cuda.synchronize()
     assert prog.last_step_fused
