@@ -246,7 +246,7 @@ int env_int(const char* k, int dflt) {
 int hall_shape_for(long long nx, long long ny, long long nz, int real_bytes) {
   const int forced = env_int("IEQ_HALL_TILE", -1);
   if (forced == 0 || forced == 1) return forced;
-  return real_bytes == 8 && 3LL * 3 * nx * ny * nz * real_bytes <= 200LL * 1000 * 1000 ? 1 : 0;
+  return real_bytes == 8 && 3LL * 3 * nx * ny * nz * real_bytes <= 1000LL * 1000 * 1000 ? 1 : 0;
 }
 
 namespace {
