@@ -46,24 +46,24 @@ template <typename R> struct PreArgs {
   double* partial;  // [gridDim.x][4] running sums of d/dc_H, d/du_x, d/du_y, d/du_z
 };
 
-template <typename R, int RR, int TY>
-__global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
+template <typename R, int RR, int TY, int TX>
+__global__ __launch_bounds__(TX * TY) void k_adj_pre(PreArgs<R> a) {
   constexpr int Q = 2 * RR + 1;
-  constexpr int LX = 64 + 2 * RR, LY = TY + 2 * RR;
+  constexpr int LX = TX + 2 * RR, LY = TY + 2 * RR, NW = TX * TY / 64;
   __shared__ R sYb[2][3][LY][LX];  // double-buffered planes: plane z in buffer (z - z0) & 1
   __shared__ R sLb[2][3][LY][LX];
-  __shared__ double red[4][64 * TY / 64];
+  __shared__ double red[4][NW];
   const int tx = threadIdx.x, ty = threadIdx.y;
   int x0, y0, z0;
-  tile_origin<TY>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
+  tile_origin<TY, TX>(xcd_tile(blockIdx.x, gridDim.x), a.nx, a.ny, a.zc, x0, y0, z0);
   const bool active = x0 + tx < a.nx && y0 + ty < a.ny;
   const long long S = a.S;
   const unsigned RB = sizeof(R);
   const unsigned colb = (unsigned)(wrap(x0 + tx, a.nx) + a.nx * wrap(y0 + ty, a.ny)) * RB;
   const unsigned plb = (unsigned)(a.nx * a.ny) * RB;
   const int zend = min(z0 + a.zc, a.nz);
-  PlaneStage<R, RR, TY, 3> hy;
-  PlaneStage<R, RR, TY, 3, 0b110, 0b101> hl;  // L: x halo L_y, L_z;  y halo L_x, L_z
+  PlaneStage<R, RR, TY, 3, 0b111, 0b111, TX> hy;
+  PlaneStage<R, RR, TY, 3, 0b110, 0b101, TX> hl;  // L: x halo L_y, L_z;  y halo L_x, L_z
   hy.init(tx, ty, x0, y0, a.nx, a.ny);
   hl.init(tx, ty, x0, y0, a.nx, a.ny);
   const Comps<R, 3> Yc = comps<R, 3>(a.Y, S), Lc = comps<R, 3>(a.L, S);
@@ -167,8 +167,8 @@ __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
       ql[1][j] = ql[1][j + 1];
     }
   }
-  // block reduction in a fixed order
-  const int lane = tx, wv = ty;
+  // block reduction in a fixed order (waves by flat thread index)
+  const int tid = tx + TX * ty, lane = tid & 63, wv = tid >> 6;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     double v = acc[j];
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(64 * TY) void k_adj_pre(PreArgs<R> a) {
   __syncthreads();
   if (ty == 0 && tx < 4) {
     double s = 0.0;
-    for (int w = 0; w < TY; ++w) s += red[tx][w];
+    for (int w = 0; w < NW; ++w) s += red[tx][w];
     a.partial[(long long)blockIdx.x * 4 + tx] += a.scale * s;
   }
 }
@@ -264,7 +264,8 @@ struct Adjoint {
   double* partial = nullptr;
   double* acc = nullptr;
   int nblk = 0;
-  int ty = 8, zc = 8;  // tile rows of k_adj_pre (64-wide tiles) and z-chunk of every launch
+  int ty = 8, zc = 8;  // tile rows of k_adj_pre and z-chunk of every launch
+  int pre_tx = 64;     // tile width of k_adj_pre
   int shape = 8;       // tile shape code of the stage and EMF launches (as in ieq_step)
   int zc_emf = 8;      // z-chunk of the EMF launches
   Binomial bin;
@@ -367,15 +368,19 @@ template <typename R> void pre(Adjoint& A, const R* Y, const R* L, double scale)
   q.G = (R*)A.G;
   q.partial = A.partial;
   const int grid = A.nblk;
-  dim3 blk(64, A.ty);
+  dim3 blk(A.pre_tx, A.ty);
   if (A.ty == 4) {
-    if (A.rr == 1) k_adj_pre<R, 1, 4><<<grid, blk, 0, A.s>>>(q);
-    else if (A.rr == 2) k_adj_pre<R, 2, 4><<<grid, blk, 0, A.s>>>(q);
-    else k_adj_pre<R, 3, 4><<<grid, blk, 0, A.s>>>(q);
+    if (A.rr == 1) k_adj_pre<R, 1, 4, 64><<<grid, blk, 0, A.s>>>(q);
+    else if (A.rr == 2) k_adj_pre<R, 2, 4, 64><<<grid, blk, 0, A.s>>>(q);
+    else k_adj_pre<R, 3, 4, 64><<<grid, blk, 0, A.s>>>(q);
+  } else if (A.pre_tx == 32) {
+    if (A.rr == 1) k_adj_pre<R, 1, 8, 32><<<grid, blk, 0, A.s>>>(q);
+    else if (A.rr == 2) k_adj_pre<R, 2, 8, 32><<<grid, blk, 0, A.s>>>(q);
+    else k_adj_pre<R, 3, 8, 32><<<grid, blk, 0, A.s>>>(q);
   } else {
-    if (A.rr == 1) k_adj_pre<R, 1, 8><<<grid, blk, 0, A.s>>>(q);
-    else if (A.rr == 2) k_adj_pre<R, 2, 8><<<grid, blk, 0, A.s>>>(q);
-    else k_adj_pre<R, 3, 8><<<grid, blk, 0, A.s>>>(q);
+    if (A.rr == 1) k_adj_pre<R, 1, 8, 64><<<grid, blk, 0, A.s>>>(q);
+    else if (A.rr == 2) k_adj_pre<R, 2, 8, 64><<<grid, blk, 0, A.s>>>(q);
+    else k_adj_pre<R, 3, 8, 64><<<grid, blk, 0, A.s>>>(q);
   }
 }
 
@@ -601,10 +606,14 @@ int adjoint_create(const Params& p, const ieq_program_info& info, long long nste
       adjoint_release(A);
       return err = "hipMalloc failed (snapshots)", IEQ_ERR_HIP;
     }
-  // tiles as in ieq_step: FP64 4 rows, FP32 8; the z-chunk that fills one round of resident
-  // workgroups of the forward stage kernel (SSPRK33 stage 2 form, convection on)
+  // tiles as in ieq_step: stage and EMF kernels FP64 32 x 8, FP32 64 x 8 where 64 divides nx and
+  // 32 x 8 otherwise (no idle lanes); k_adj_pre FP64 64 x 4, FP32 64 x 8 or 32 x 8 alike; the
+  // z-chunk that fills one round of resident workgroups of the forward stage kernel (SSPRK33
+  // stage 2 form, convection on)
+  const bool narrow = info.real_bytes == 4 && p.nx % 64 != 0;
   A->ty = info.real_bytes == 8 ? 4 : 8;
-  A->shape = info.real_bytes == 8 ? 32 : 8;
+  A->pre_tx = narrow ? 32 : 64;
+  A->shape = info.real_bytes == 8 || narrow ? 32 : 8;
   {
     int fake[3];  // distinct addresses standing for b1, b2, E: only compared, never read
     if (info.real_bytes == 8) {
@@ -629,7 +638,7 @@ int adjoint_create(const Params& p, const ieq_program_info& info, long long nste
       A->zc_emf = pick_zc_emf<float>(A->rr, A->shape, t);
     }
   }
-  A->nblk = ((p.nx + 63) / 64) * ((p.ny + A->ty - 1) / A->ty) * ((p.nz + A->zc - 1) / A->zc);
+  A->nblk = ((p.nx + A->pre_tx - 1) / A->pre_tx) * ((p.ny + A->ty - 1) / A->ty) * ((p.nz + A->zc - 1) / A->zc);
   if (hipMalloc(&A->partial, (size_t)A->nblk * 4 * sizeof(double)) != hipSuccess ||
       hipMalloc(&A->acc, 4 * sizeof(double)) != hipSuccess) {
     adjoint_release(A);

@@ -246,7 +246,8 @@ int env_int(const char* k, int dflt) {
 int hall_shape_for(long long nx, long long ny, long long nz, int real_bytes) {
   const int forced = env_int("IEQ_HALL_TILE", -1);
   if (forced == 0 || forced == 1) return forced;
-  return real_bytes == 8 && 3LL * 3 * nx * ny * nz * real_bytes <= 1000LL * 1000 * 1000 ? 1 : 0;
+  if (real_bytes == 4) return nx % 64 != 0 ? 1 : 0;  // 32 x 8 where 64-wide tiles would leave lanes idle
+  return 3LL * 3 * nx * ny * nz * real_bytes <= 1000LL * 1000 * 1000 ? 1 : 0;
 }
 
 namespace {
@@ -500,10 +501,11 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
   a.ny = p.ny;
   a.nz = p.nz;
   // Tile shape measured at 128^3: FP64 32 x 8 (2 waves per SIMD, so small workgroups; the
-  // narrow tile keeps the y halo a smaller share of the loads than 64 x 4), FP32 64 x 8.
-  // The z-chunk is set below from the stage kernel's occupancy.
+  // narrow tile keeps the y halo a smaller share of the loads than 64 x 4), FP32 64 x 8 where
+  // 64 divides nx and 32 x 8 otherwise (no idle lanes: 96^3 SSPRK33 with convection, 33.6 ->
+  // 28.5 us per stage).  The z-chunk is set below from the stage kernel's occupancy.
   const bool f64 = sizeof(R) == 8;
-  const int ty_env = env_int("IEQ_FUSED_TY", f64 ? 32 : 8);  // tile shape code: 4, 8 or 32 (32 x 8)
+  const int ty_env = env_int("IEQ_FUSED_TY", f64 || p.nx % 64 != 0 ? 32 : 8);  // tile shape code: 4, 8 or 32 (32 x 8)
   const int ty = ty_env == 4 || ty_env == 32 ? ty_env : 8;
   a.S = info.stride;
   a.c = coef;
@@ -548,7 +550,10 @@ int fused_step_t(const Params& p, const ieq_program_info& info, hipStream_t s, F
   }
   // One kernel per Hall stage unless split on request or the 32-bit byte offsets of the staged
   // halo (which include the component) would not fit.
-  const bool one_kernel = p.hall && env_int("IEQ_FUSED_SPLIT", 0) == 0 && (long long)3 * info.stride * sizeof(R) < (1LL << 32);
+  // FP32 stages with convection run faster as the EMF kernel + stage kernel pair (96^3 SSPRK33:
+  // 28.5 against 35.2 us per stage; 128^3: equal), every other Hall stage as one kernel.
+  const int split = env_int("IEQ_FUSED_SPLIT", (!f64 && a.conv) ? 1 : 0);
+  const bool one_kernel = p.hall && split == 0 && (long long)3 * info.stride * sizeof(R) < (1LL << 32);
   int zc_emf = 0, zc_hall = 0;
   if (p.hall && !one_kernel) {
     zc_emf = env_int("IEQ_EMF_ZC", 0);

@@ -20,7 +20,8 @@ template <typename R> int pick_zc_hall(int rr, const FArgs<R>& a, bool hod, int 
 // Tile of single-grid launches for an nx x ny x nz grid of `real_bytes` REALs (IEQ_HALL_TILE=0/1
 // overrides): in FP64 16 x 16 up to 1 GB for the three vector fields a 2N stage touches (128^3:
 // 66.3 against 67.9 us per stage, 160^3-224^3: 1-2 % faster), 32 x 8 beyond (256^3: 436.7
-// against 449.3 us).
+// against 449.3 us); in FP32 64 x 8 where 64 divides nx, 32 x 8 otherwise (96^3: 35.2 against
+// 39.0 us).
 int hall_shape_for(long long nx, long long ny, long long nz, int real_bytes);
 
 }  // namespace ieq
