@@ -101,6 +101,21 @@ def test_ensemble_members_match_oracle(integrator, ops, hall, diss, shape, monke
         assert err <= 1e-12, "member %d: rel err %.3e" % (i, err)
 
 
+def test_tuner_overrides_pin_the_launch(monkeypatch):
+    """IEQ_ENS_SHAPE / IEQ_ENS_ZC / IEQ_ENS_ORDER restrict the tuner to one launch form, z-chunk and tile order;
+    the pinned launch reports them and its members still equal the oracle (FP64, 1e-12)."""
+    monkeypatch.setenv("IEQ_ENS_SHAPE", "2")
+    monkeypatch.setenv("IEQ_ENS_ZC", "8")
+    monkeypatch.setenv("IEQ_ENS_ORDER", "1")
+    cfgs, ens, Bs = _ensemble("6thOrder", True, "HIGH_ORDER", "SSPRK33", "double", seeds=(2, 3, 4))
+    ens.step(2)
+    c = ens.config()
+    assert (c["tile"], c["z_chunk"], c["order"]) == ("hall_16x16", 8, "largest_first"), c
+    for i, (cfg, B) in enumerate(zip(cfgs, Bs)):
+        want = _oracle_steps(cfg, "SSPRK33", B, MEMBERS[i][3], 2)
+        assert np.abs(ens.state(i).numpy() - want).max() / np.abs(want).max() <= 1e-12
+
+
 def test_ensemble_repeated_steps_continue_the_solve():
     """Two calls of 2 steps equal one oracle run of 4 steps (the graph replays from b1)."""
     cfgs, ens, Bs = _ensemble("6thOrder", True, "HIGH_ORDER", "CarpenterKennedy2N54", "double", seeds=(4, 5, 6))
