@@ -27,6 +27,10 @@ namespace {
 #ifndef IEQ_HALL_HSKIP
 #define IEQ_HALL_HSKIP 0
 #endif
+// E at the E-halo columns only on the planes the chunk outputs (1), or on every plane (0).
+#ifndef IEQ_HALL_ZSKIP
+#define IEQ_HALL_ZSKIP 1
+#endif
 // Extra LDS row elements (bank spreading of the scattered halo accesses); 0 by default.
 #ifndef IEQ_HALL_PADB
 #define IEQ_HALL_PADB 0
@@ -320,7 +324,12 @@ __device__ __forceinline__ void hall_stage_tile(const FArgs<R>& a, int x0, int y
         }
       }
     }
-    // E at the E-halo column (a wave whose threads all lie past the last halo column skips it)
+    // E at the E-halo column: only planes whose x/y derivatives of E the chunk takes (the z lead-in
+    // and tail planes feed the own column's z-derivatives alone); a wave whose threads all lie past
+    // the last halo column may skip it too
+#if IEQ_HALL_ZSKIP
+    if (rhs_pe)
+#endif
 #if IEQ_HALL_HSKIP
     if (has_h)
 #endif
