@@ -3,7 +3,8 @@
 * cfg1 at its full size: one CarpenterKennedy2N54 step of the periodic 128^3 Hall problem
   (6th order + HO dissipation, u = 0, rho = 1, B0 = (0,0,1) + random modes |k| <= 4, seed 1)
   through ieq_step, which picks the production tile shape and occupancy-sized z-chunks;
-  FP64 against the FP64 oracle to 1e-12, FP32 against the FP32 oracle to 2e-5.
+  FP64 against the FP64 oracle to 1e-12, FP32 against the FP32 oracle to 2e-5; the same step at
+  256^3 in FP64, where ieq_step takes 32 x 8 tiles and 128-plane z-chunks.
 * forced z-chunks (IEQ_FUSED_ZC / IEQ_EMF_ZC = 2, 3, 4) so every tile runs the multi-chunk
   lead-in and a partial last chunk, for both tiles of the one-kernel stage (IEQ_HALL_TILE).
 * the exact solution B0 + eps*b(x - (u + m B0/rho) t) of the implemented equation for an ABC
@@ -122,6 +123,34 @@ def test_hall128_step_matches_oracle(real, tile):
     # the step moved the state by far more than the disagreement
     assert _relerr(want, B0) > 10 * e
     assert abs(float(args[-1][0]) - float(ht[0])) <= (1e-15 if real == "double" else 1e-7)
+
+
+def test_hall256_step_matches_oracle():
+    """The 256^3 FP64 Hall stage at its production launch (32 x 8 tiles, two 128-plane z-chunks per tile
+    column, the shape ieq_step picks once the grid leaves the cache): one CK2N54 step vs the oracle.  The
+    initial field has every random mode with |k| <= 2 (full 3-D variation, cheap to build at this size)."""
+    import torch
+
+    from inductioneq_amd.program import Program
+
+    n = 256
+    cfg = _periodic_cfg(n, "double", w_size=256)
+    B0 = random_modes_field(n, seed=5, kmax=2, amp=0.3)
+    cfg.dt = hall_dt(B0, (cfg.dx, cfg.dy, cfg.dz), 6, c_hall=1.0, cfl=0.5)
+    prog = Program(cfg)
+    args = _device_args(prog, B0, 2, (0.0, 0.0, 0.0), 1.0)
+    prog.step("CarpenterKennedy2N54", 1, args)
+    torch.cuda.synchronize()
+    assert prog.last_step_fused
+    got = args[0][:, : cfg.num_nodes].double().cpu().numpy().T
+    del args, prog
+    torch.cuda.empty_cache()
+    hb, ht = _oracle_steps(cfg, B0, "CarpenterKennedy2N54", 1, (0.0, 0.0, 0.0), 1.0)
+    want = hb[0][: cfg.num_nodes, :3].astype(np.float64)
+    e = _relerr(got, want)
+    print("hall256 double one CK2N54 step: rel err vs oracle %.3e" % e)
+    assert e <= TOL["double"], e
+    assert _relerr(want, B0) > 10 * e
 
 
 @pytest.mark.parametrize("split,tile", [(0, 0), (0, 1), (1, 0)])
