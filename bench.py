@@ -114,7 +114,7 @@ def main():
         print(json.dumps(traffic_summary(*a.traffic_from), indent=1))
         return
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU only (the workload is single-GPU; multi-GPU domain "
+        print("bench.py: measures one GPU only (the workload is single-GPU; multi-GPU domain "
               "decomposition is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if a.workload == "ensemble64":
@@ -259,7 +259,7 @@ def fp32_vs_fp64(cfg, B0, integrator, nsteps, b1_32):
 
 
 def run_secondary(a):
-    """The other BASELINE measurements, each on its own configuration: the Hall stage at 256^3 FP64 (beyond the
+    """The secondary measurements, each on its own configuration: the Hall stage at 256^3 FP64 (beyond the
     256 MB cache) and at 128^3 FP32, the batched ensemble against per-instance FP64 solves, the adjoint gradient
     cost at 96^3 x 200 steps with 15 binomial snapshots, and CG divergence cleaning at 192^3."""
     import torch

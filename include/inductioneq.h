@@ -35,7 +35,7 @@ enum {
   IEQ_ERR_KERNEL = -2,      /* kernel name not in the program (CL_INVALID_KERNEL_NAME) */
   IEQ_ERR_ARGS = -3,        /* wrong argument count, null pointer or NDRange outside the grid */
   IEQ_ERR_HIP = -4,         /* HIP runtime error */
-  IEQ_ERR_UNSUPPORTED = -5  /* option combination this build does not provide */
+  IEQ_ERR_UNSUPPORTED = -5  /* option combination this library does not provide */
 };
 
 /* Parsed program settings (read-only view). */
